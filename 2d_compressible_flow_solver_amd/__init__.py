@@ -1,0 +1,340 @@
+"""Python host side of the MI355X per-stage hot path (ctypes over libcfs.so).
+
+The C-ABI is include/cfs.h.  This module mirrors the reference's driver-level
+objects for scripting and tests:
+
+  Config / config_*()       the mesh + plugin choices of Run::initialize
+                            (Run.cpp:10-90) and param.h, as one cfs_config
+  Solver                    a device-resident solve; stage()/step() replace one
+                            iteration / one RK step of Run::solver (Run.cpp:153-204)
+  circle_setup()            CartesianMesh::setImmersedBoundary (CartesianMesh.cpp:357-374)
+  riemann_quadrants(), cylinder_freestream(), kelvin_helmholtz()
+                            initial conditions (Initialcondition::fillIn,
+                            source/initialcondition/*.cpp), host-side inputs
+
+The package directory name starts with a digit, so load it with
+importlib.import_module("2d_compressible_flow_solver_amd").
+
+There is no CPU fallback: every compute call goes to the HIP library and
+raises CfsError when it is missing or a HIP call fails.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcfs.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "cfs.h")
+
+CFS_OK = 0
+CFS_E_ARG = -1
+CFS_E_HIP = -2
+CFS_E_OOM = -3
+CFS_E_STATE = -4
+CFS_E_NONFINITE = -5
+CFS_E_UNSUPPORTED = -6
+
+BC_NEUMANN, BC_DIRICHLET, BC_PERIODIC = 0, 1, 2
+SIDE_LEFT, SIDE_RIGHT, SIDE_BOTTOM, SIDE_TOP = 0, 1, 2, 3
+PREC_FP64, PREC_FP32_BULK = 0, 1
+CELL_FLUID, CELL_SOLID, CELL_IMMERSED_GHOST = 0, 1, 2
+IB_REFERENCE, IB_BILINEAR_POSITIVE = 0, 1
+
+
+class CfsError(RuntimeError):
+    def __init__(self, status, what=""):
+        self.status = status
+        msg = _lib().cfs_status_string(status).decode() if _LIB is not None else str(status)
+        super().__init__("%s%s" % (what + ": " if what else "", msg))
+
+
+class Config(ctypes.Structure):
+    """cfs_config of include/cfs.h (same field order and types)."""
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
+        ("lx", ctypes.c_double), ("ly", ctypes.c_double),
+        ("x0", ctypes.c_double), ("y0", ctypes.c_double),
+        ("gamma", ctypes.c_double),
+        ("bc", ctypes.c_int32 * 4),
+        ("bc_state", (ctypes.c_double * 4) * 4),
+        ("rk", ctypes.c_int32), ("precision", ctypes.c_int32), ("viscous", ctypes.c_int32),
+        ("positivity", ctypes.c_int32), ("hll_bounds", ctypes.c_int32),
+        ("mu", ctypes.c_double), ("prandtl", ctypes.c_double), ("ma_ref", ctypes.c_double),
+    ]
+
+    @property
+    def dx(self):
+        return self.lx / self.nx
+
+    @property
+    def dy(self):
+        return self.ly / self.ny
+
+    def centres(self):
+        """Cell centres ((i+0.5)*dx + x0, (j+0.5)*dy + y0), arrays [nx][ny]
+        (CartesianMesh::SetElement, CartesianMesh.cpp:82-97)."""
+        i = np.arange(self.nx, dtype=np.float64)
+        j = np.arange(self.ny, dtype=np.float64)
+        x = (i + 0.5) * self.dx + self.x0
+        y = (j + 0.5) * self.dy + self.y0
+        return np.meshgrid(x, y, indexing="ij")
+
+
+def make_config(nx, ny, lx, ly, bc=(BC_NEUMANN,) * 4, bc_state=None, gamma=1.4, rk=3,
+                precision=PREC_FP64, x0=0.0, y0=0.0, viscous=0, mu=0.0, prandtl=0.72, ma_ref=1.0,
+                positivity=0, hll_bounds=0):
+    c = Config()
+    c.nx, c.ny, c.lx, c.ly, c.x0, c.y0, c.gamma = nx, ny, lx, ly, x0, y0, gamma
+    for k in range(4):
+        c.bc[k] = bc[k]
+        st = (bc_state[k] if bc_state is not None and bc_state[k] is not None else (0.0, 0.0, 0.0, 0.0))
+        for q in range(4):
+            c.bc_state[k][q] = st[q]
+    c.rk, c.precision, c.viscous, c.positivity, c.hll_bounds = rk, precision, viscous, positivity, hll_bounds
+    c.mu, c.prandtl, c.ma_ref = mu, prandtl, ma_ref
+    return c
+
+
+def conserved(rho, u, v, p, gamma=1.4):
+    """(rho, u, v, p) -> (rho, rho*u, rho*v, E), E by EulerPhase::computeEnergyfromOther
+    (EulerPhase.cpp:124-128): e = p/(gamma-1); E = e + 0.5*(mu*mu + mv*mv)/rho."""
+    rho = np.asarray(rho, dtype=np.float64)
+    mu = rho * np.asarray(u, dtype=np.float64)
+    mv = rho * np.asarray(v, dtype=np.float64)
+    e = np.asarray(p, dtype=np.float64) / (gamma - 1)
+    E = e + 0.5 * (mu * mu + mv * mv) / rho
+    return rho, mu, mv, E
+
+
+# ---------------------------------------------------------------------------
+# named workloads (the configurations the hot path is measured and tested on)
+
+LAX_LIU_3 = {  # (rho, u, v, p) per quadrant
+    "NE": (1.5, 0.0, 0.0, 1.5),
+    "NW": (0.5323, 1.206, 0.0, 0.3),
+    "SW": (0.138, 1.206, 1.206, 0.029),
+    "SE": (0.5323, 0.0, 1.206, 0.3),
+}
+
+
+def riemann_quadrants(cfg, states=LAX_LIU_3, split=(0.5, 0.5)):
+    """Four-quadrant Riemann initial state [4][nx][ny] (Reimman2D::fillIn,
+    Reimman2D.cpp:5-39, with the split point as a parameter: the reference
+    hard-codes 1.6)."""
+    x, y = cfg.centres()
+    sx, sy = split
+    U = np.empty((4, cfg.nx, cfg.ny))
+    for key, m in (("SW", (x < sx) & (y < sy)), ("NW", (x < sx) & (y >= sy)),
+                   ("SE", (x >= sx) & (y < sy)), ("NE", (x >= sx) & (y >= sy))):
+        cons = conserved(*states[key], gamma=cfg.gamma)
+        for q in range(4):
+            U[q][m] = cons[q]
+    return U
+
+
+def config_riemann(n=400, rk=3, precision=PREC_FP64):
+    """Lax-Liu configuration 3 on [0,1]^2, transmissive boundaries."""
+    return make_config(n, n, 1.0, 1.0, bc=(BC_NEUMANN,) * 4, rk=rk, precision=precision)
+
+
+CYLINDER_FREESTREAM = (1.4, 3.0, 0.0, 1.0)   # rho, u, v, p: Mach 3 at c = 1
+
+
+def config_cylinder(nx=2048, ny=1024, lx=4.0, ly=2.0, freestream=CYLINDER_FREESTREAM, rk=3,
+                    precision=PREC_FP64, gamma=1.4, positivity=1, hll_bounds=1):
+    """Supersonic inflow on the left (Dirichlet), extrapolation elsewhere
+    (Cylinder::setBounbdaryCondition, Cylinder.cpp:19-40)."""
+    inflow = tuple(float(a) for a in conserved(*freestream, gamma=gamma))
+    return make_config(nx, ny, lx, ly, bc=(BC_DIRICHLET, BC_NEUMANN, BC_NEUMANN, BC_NEUMANN),
+                       bc_state=(inflow, None, None, None), rk=rk, precision=precision, gamma=gamma,
+                       positivity=positivity, hll_bounds=hll_bounds)
+
+
+def cylinder_freestream(cfg, cell_type, freestream=CYLINDER_FREESTREAM):
+    """Freestream on fluid cells, zero elsewhere (Cylinder::fillIn applied to
+    fluid cells only, Run.cpp:99-105)."""
+    cons = conserved(*freestream, gamma=cfg.gamma)
+    U = np.zeros((4, cfg.nx, cfg.ny))
+    fluid = cell_type.reshape(cfg.nx, cfg.ny) == CELL_FLUID
+    for q in range(4):
+        U[q][fluid] = cons[q]
+    return U
+
+
+# ---------------------------------------------------------------------------
+_LIB = None
+
+
+def _lib():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libcfs.so is not built at %s (run __graft_entry__.build())" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    dbl, i32 = ctypes.c_double, ctypes.c_int32
+    vp = ctypes.c_void_p
+    lib.cfs_abi_version.restype = i32
+    lib.cfs_status_string.restype = ctypes.c_char_p
+    lib.cfs_status_string.argtypes = [i32]
+    lib.cfs_device_count.restype = i32
+    lib.cfs_create.argtypes = [P(Config), vp, P(vp)]
+    lib.cfs_destroy.argtypes = [vp]
+    lib.cfs_destroy.restype = None
+    for name in ("cfs_set_state", "cfs_get_state", "cfs_set_state_device", "cfs_get_state_device"):
+        getattr(lib, name).argtypes = [vp, vp]
+    lib.cfs_circle_setup.argtypes = [P(Config), dbl, dbl, dbl, i32, i32, i32, i32, vp, P(i32), vp, vp, vp, vp,
+                                     vp]
+    lib.cfs_set_immersed.argtypes = [vp, i32, vp, i32, vp, vp, vp, vp, vp]
+    lib.cfs_stage.argtypes = [vp, i32, dbl]
+    lib.cfs_step.argtypes = [vp, dbl]
+    lib.cfs_advance.argtypes = [vp, i32, dbl, dbl, dbl, P(dbl), P(i32)]
+    lib.cfs_compute_dt.argtypes = [vp, dbl, P(dbl)]
+    lib.cfs_checksums.argtypes = [vp, vp]
+    lib.cfs_stage_counter.argtypes = [vp]
+    lib.cfs_stage_counter.restype = i32
+    lib.cfs_last_kernel_ms.argtypes = [vp, P(dbl)]
+    for name in ("cfs_create", "cfs_set_state", "cfs_get_state", "cfs_set_state_device",
+                 "cfs_get_state_device", "cfs_circle_setup", "cfs_set_immersed", "cfs_stage",
+                 "cfs_step", "cfs_advance", "cfs_compute_dt", "cfs_checksums", "cfs_last_kernel_ms"):
+        getattr(lib, name).restype = i32
+    _LIB = lib
+    return lib
+
+
+def library():
+    """The loaded ctypes library (raises if libcfs.so is not built)."""
+    return _lib()
+
+
+def _check(rc, what):
+    if rc != CFS_OK:
+        raise CfsError(rc, what)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def device_count():
+    return _lib().cfs_device_count()
+
+
+def circle_setup(cfg, cx, cy, radius, n_lagrangian=400, n_layers=3, interp=IB_BILINEAR_POSITIVE):
+    """Immersed-boundary tables of a circle (host code of libcfs.so; no GPU needed).
+    interp: IB_REFERENCE (the reference's LDLT-on-lower-triangle interpolation,
+    as written) or IB_BILINEAR_POSITIVE (exact bilinear fit when non-degenerate
+    and non-negative, else inverse-distance weights).
+    Returns dict(interp, cell_type, n_ig, ig_cells, donors, image_xy, coef, transp)."""
+    lib = _lib()
+    n = cfg.nx * cfg.ny
+    cell_type = np.zeros(n, dtype=np.uint8)
+    n_ig = ctypes.c_int32(0)
+    z = np.zeros(1, dtype=np.int32)
+    rc = lib.cfs_circle_setup(ctypes.byref(cfg), cx, cy, radius, n_lagrangian, n_layers, interp, 0,
+                              _ptr(cell_type), ctypes.byref(n_ig), None, None, None, None, None)
+    if rc not in (CFS_OK, CFS_E_ARG):
+        _check(rc, "cfs_circle_setup")
+    cap = max(n_ig.value, 1)
+    out = dict(interp=interp, cell_type=cell_type,
+               ig_cells=np.zeros(cap, dtype=np.int32), donors=np.zeros(4 * cap, dtype=np.int32),
+               image_xy=np.zeros(2 * cap), coef=np.zeros(10 * cap), transp=np.zeros(4 * cap, dtype=np.int32))
+    rc = lib.cfs_circle_setup(ctypes.byref(cfg), cx, cy, radius, n_lagrangian, n_layers, interp, cap,
+                              _ptr(cell_type), ctypes.byref(n_ig), _ptr(out["ig_cells"]), _ptr(out["donors"]),
+                              _ptr(out["image_xy"]), _ptr(out["coef"]), _ptr(out["transp"]))
+    _check(rc, "cfs_circle_setup")
+    k = n_ig.value
+    out["n_ig"] = k
+    for key, w in (("ig_cells", 1), ("donors", 4), ("image_xy", 2), ("coef", 10), ("transp", 4)):
+        out[key] = out[key][: w * k]
+    del z
+    return out
+
+
+class Solver:
+    """One device-resident solve (a cfs_solver handle)."""
+
+    def __init__(self, cfg, stream=None):
+        self.cfg = cfg
+        self._lib = _lib()
+        h = ctypes.c_void_p()
+        _check(self._lib.cfs_create(ctypes.byref(cfg), stream, ctypes.byref(h)), "cfs_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.cfs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def shape(self):
+        return (4, self.cfg.nx, self.cfg.ny)
+
+    def set_state(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        if U.shape != self.shape:
+            raise ValueError("state must be %s, got %s" % (self.shape, U.shape))
+        _check(self._lib.cfs_set_state(self._h, _ptr(U)), "cfs_set_state")
+
+    def get_state(self):
+        U = np.empty(self.shape)
+        _check(self._lib.cfs_get_state(self._h, _ptr(U)), "cfs_get_state")
+        return U
+
+    def set_state_device(self, ptr):
+        _check(self._lib.cfs_set_state_device(self._h, ctypes.c_void_p(ptr)), "cfs_set_state_device")
+
+    def get_state_device(self, ptr):
+        _check(self._lib.cfs_get_state_device(self._h, ctypes.c_void_p(ptr)), "cfs_get_state_device")
+
+    def set_immersed(self, ib):
+        _check(self._lib.cfs_set_immersed(self._h, int(ib["interp"]), _ptr(ib["cell_type"]), int(ib["n_ig"]),
+                                          _ptr(ib["ig_cells"]), _ptr(ib["donors"]), _ptr(ib["image_xy"]),
+                                          _ptr(ib["coef"]), _ptr(ib["transp"])), "cfs_set_immersed")
+
+    @property
+    def counter(self):
+        return self._lib.cfs_stage_counter(self._h)
+
+    def stage(self, counter, dt):
+        _check(self._lib.cfs_stage(self._h, counter, dt), "cfs_stage")
+
+    def step(self, dt):
+        _check(self._lib.cfs_step(self._h, dt), "cfs_step")
+
+    def advance(self, nsteps, dt=0.0, cfl=0.5, t_end=0.0, t=0.0):
+        """nsteps RK steps; dt > 0 fixed, else CFL.  Returns (t, steps_done)."""
+        tt = ctypes.c_double(t)
+        done = ctypes.c_int32(0)
+        _check(self._lib.cfs_advance(self._h, nsteps, dt, cfl, t_end, ctypes.byref(tt), ctypes.byref(done)),
+               "cfs_advance")
+        return tt.value, done.value
+
+    def compute_dt(self, cfl):
+        dt = ctypes.c_double(0.0)
+        _check(self._lib.cfs_compute_dt(self._h, cfl, ctypes.byref(dt)), "cfs_compute_dt")
+        return dt.value
+
+    def checksums(self):
+        out = np.zeros(4)
+        _check(self._lib.cfs_checksums(self._h, _ptr(out)), "cfs_checksums")
+        return out
+
+    def last_kernel_ms(self):
+        ms = ctypes.c_double(0.0)
+        _check(self._lib.cfs_last_kernel_ms(self._h, ctypes.byref(ms)), "cfs_last_kernel_ms")
+        return ms.value

@@ -1,0 +1,59 @@
+"""Builds libcfs.so (HIP kernels + C-ABI) in-tree for gfx950 with hipcc.
+
+Usage: python 2d_compressible_flow_solver_amd/build.py [-v]
+
+Flags that matter for results:
+  -ffp-contract=off   no FMA contraction: FP64 kernels must reproduce the
+                      reference's separate multiply/add roundings;
+  -Xclang -target-feature -Xclang -packed-fp32-ops
+                      no packed FP32 instructions (they have given wrong
+                      results on these machines).
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libcfs.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
+DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-munsafe-fp-atomics"]
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines()
+                    if "packed-fp32-ops" not in l and l.strip())
+    if r.returncode != 0:
+        raise RuntimeError("build failed: %s\n%s" % (" ".join(cmd), r.stdout))
+    if verbose and out:
+        print(out)
+
+
+def build(verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    objs = []
+    for src in ["cfs_api.hip"]:
+        obj = os.path.join(BUILD, src + ".o")
+        _run([HIPCC, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj] + COMMON + DEVICE, verbose)
+        objs.append(obj)
+    for src in ["ib_setup.cpp"]:
+        obj = os.path.join(BUILD, src + ".o")
+        _run(["g++", "-c", os.path.join(CSRC, src), "-o", obj] + COMMON, verbose)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", "-o", tmp] + objs + ["--offload-arch=" + ARCH, "-fPIC"], verbose)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,487 @@
+// kernels.hpp -- gfx950 kernels of the per-stage hot path.
+//
+// Layout in HBM: the conserved state is three FP64 plane-sets A, B, C, each
+// [4][nx][ny] (x-major, j contiguous), used as RK ping-pong buffers; an
+// optional uint8 cell-type plane [nx][ny]; immersed-boundary tables.
+//
+// Stage kernel (one launch per RK stage): a 256-thread workgroup owns a
+// TI x TJ = 15 x 16 cell tile.  It stages the tile plus a 3-cell halo
+// (the WENO5 stencil reach, minus the unused corners) in LDS, applying the
+// physical boundary on the fly (one ghost cell per boundary face, as
+// source/boundary/*Boundary.cpp; periodic wrap), then
+//   phase 1: every thread computes one or two faces -- the 16x16 = 256
+//            x-faces and the 15x17 = 255 y-faces of the tile, 511 in all,
+//            two rounds of 256 with one idle slot -- WENO5 left/right
+//            states, Euler fluxes and the HLL flux, in the face-normal frame
+//            so x and y faces run the same code with no divergence; fluxes
+//            go to LDS;
+//   phase 2: every thread updates one cell: flux divergence and the TVD-RK
+//            combination, written to the next plane.
+// Every face is computed exactly once per tile (tile-edge faces once per
+// adjacent tile: 511 faces per 240 cells).
+//
+// Arithmetic: FP64 kernels reproduce the reference's operations one for one
+// in its source order (compiled with -ffp-contract=off; IEEE division and
+// square root), so their results equal a strict-IEEE CPU evaluation of the
+// reference source bit for bit.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cfs {
+
+constexpr int TI = 15;             // tile rows (x / i direction)
+constexpr int TJ = 16;             // tile columns (y / j direction, contiguous)
+constexpr int HALO = 3;            // WENO5 reach: stencil offsets -3..+2
+constexpr int SR = TI + 2 * HALO;  // 21 LDS rows
+constexpr int SC = TJ + 2 * HALO;  // 22 LDS columns
+constexpr int NXF = (TI + 1) * TJ; // 256 x faces per tile
+constexpr int NYF = TI * (TJ + 1); // 255 y faces per tile
+constexpr int BLOCK = 256;
+
+enum { BC_NEUMANN = 0, BC_DIRICHLET = 1, BC_PERIODIC = 2 };
+
+struct StageArgs {
+    const double* in;     // state being differentiated [4][nx][ny]
+    const double* u0;     // RK snapshot U0 (forms 1, 2)
+    const double* u1;     // RK snapshot U1 (form 2)
+    double* out;          // next state
+    const uint8_t* type;  // cell types or nullptr (all fluid)
+    const double* dt;     // device scalar: time step of this stage
+    long long plane;      // nx*ny
+    int nx, ny;
+    int tiles_j;          // tiles along j
+    int form;             // 0: c1*in + (c2*dt)L   1: (c0*u0 + c1*in) + (c2*dt)L   2: (c0*u0 + c1*u1) + (c2*dt)L
+    int positivity;       // inadmissible WENO face state -> adjacent cell state
+    int hll_bounds;       // 0: reference (momentum, left state in SR), 1: velocity bounds
+    double dx, dy, gamma;
+    double c0, c1, c2;
+    int bc[4];
+    double bcs[4][4];
+};
+
+// ---------------------------------------------------------------------------
+// scalar building blocks (templated on the reconstruction/flux type)
+
+template <typename R> struct K;
+template <> struct K<double> {
+    static constexpr double eps = 1e-2, c1312 = 13.0 / 12.0, q = 0.25;
+    static constexpr double d0 = 0.1, d1 = 0.6, d2 = 0.3, half = 0.5;
+    __device__ static double div(double a, double b) { return a / b; }
+    __device__ static double sqrt_(double a) { return __builtin_sqrt(a); }
+};
+template <> struct K<float> {
+    static constexpr float eps = 1e-2f, c1312 = 13.0f / 12.0f, q = 0.25f;
+    static constexpr float d0 = 0.1f, d1 = 0.6f, d2 = 0.3f, half = 0.5f;
+    // FP32-bulk: hardware reciprocal (1 ulp) instead of the IEEE division
+    __device__ static float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    __device__ static float sqrt_(float a) { return __builtin_amdgcn_sqrtf(a); }
+};
+
+// WENO5 value at the face from five cell values, v2 the upwind cell
+// (source/interpolation/Weno5th.cpp:4-28: Jiang-Shu smoothness indicators,
+// eps = 1e-2, ideal weights 0.1/0.6/0.3).  Operation order as the reference.
+template <typename R>
+__device__ __forceinline__ R weno5(R v0, R v1, R v2, R v3, R v4) {
+    using C = K<R>;
+    const R s0 = (v0 - R(2) * v1) + v2;
+    const R t0 = (v0 - R(4) * v1) + R(3) * v2;
+    const R b0 = C::c1312 * (s0 * s0) + C::q * (t0 * t0);
+    const R s1 = (v1 - R(2) * v2) + v3;
+    const R t1 = v1 - v3;
+    const R b1 = C::c1312 * (s1 * s1) + C::q * (t1 * t1);
+    const R s2 = (v2 - R(2) * v3) + v4;
+    const R t2 = (R(3) * v2 - R(4) * v3) + v4;
+    const R b2 = C::c1312 * (s2 * s2) + C::q * (t2 * t2);
+    const R e0 = C::eps + b0, e1 = C::eps + b1, e2 = C::eps + b2;
+    const R a0 = C::div(C::d0, e0 * e0);
+    const R a1 = C::div(C::d1, e1 * e1);
+    const R a2 = C::div(C::d2, e2 * e2);
+    const R sum = (a0 + a1) + a2;
+    const R w0 = C::div(a0, sum), w1 = C::div(a1, sum), w2 = C::div(a2, sum);
+    const R q0 = (C::div(v0, R(3)) - C::div(R(7) * v1, R(6))) + C::div(R(11) * v2, R(6));
+    const R q1 = (C::div(-v1, R(6)) + C::div(R(5) * v2, R(6))) + C::div(v3, R(3));
+    const R q2 = (C::div(v2, R(3)) + C::div(R(5) * v3, R(6))) - C::div(v4, R(6));
+    return (w0 * q0 + w1 * q1) + w2 * q2;
+}
+
+// Face flux from the 6-cell stencil in the face-normal frame.
+// st[k][0..3] = (rho, normal momentum, tangential momentum, E) of stencil
+// cell k (offsets -3..+2).  Left state: WENO over cells 0..4; right state:
+// over cells 5..1 (Weno5th.cpp:57-81).  Pressure (EulerPhase.cpp:130-133),
+// Euler flux (EulerMod.cpp:33-54) and HLL (RiemmanSolverHLL.cpp:5-75) follow.
+// The x and y forms of the reference map onto this one form because IEEE
+// addition and multiplication are commutative (mx*my == my*mx bit for bit).
+// out = (mass, normal momentum, tangential momentum, energy) flux.
+template <typename R>
+__device__ __forceinline__ void face_flux(const R* __restrict__ S, const int cell[6], int nrm, int tng,
+                                          R g, int positivity, int hll_bounds, R out[4]) {
+    using C = K<R>;
+    R L[4], Rr[4], cl[4], cr[4];
+    const int comp[4] = {0, nrm, tng, 3};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        R v[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) v[k] = S[cell[k] + comp[q]];
+        L[q] = weno5<R>(v[0], v[1], v[2], v[3], v[4]);
+        Rr[q] = weno5<R>(v[5], v[4], v[3], v[2], v[1]);
+        cl[q] = v[2];
+        cr[q] = v[3];
+    }
+    const R gm1 = g - R(1);
+    R rL = L[0], nL = L[1], tL = L[2], EL = L[3];
+    R rR = Rr[0], nR = Rr[1], tR = Rr[2], ER = Rr[3];
+    R pL = gm1 * (EL - C::div(C::half * (nL * nL + tL * tL), rL));
+    R pR = gm1 * (ER - C::div(C::half * (nR * nR + tR * tR), rR));
+    if (positivity) {
+        // an inadmissible reconstruction takes the state of the cell on its side
+        if (!(rL > R(0) && pL > R(0))) {
+            rL = cl[0]; nL = cl[1]; tL = cl[2]; EL = cl[3];
+            pL = gm1 * (EL - C::div(C::half * (nL * nL + tL * tL), rL));
+        }
+        if (!(rR > R(0) && pR > R(0))) {
+            rR = cr[0]; nR = cr[1]; tR = cr[2]; ER = cr[3];
+            pR = gm1 * (ER - C::div(C::half * (nR * nR + tR * tR), rR));
+        }
+    }
+    // Euler fluxes of both states
+    const R unL = C::div(nL, rL), unR = C::div(nR, rR);
+    const R FL0 = nL, FL1 = C::div(nL * nL, rL) + pL, FL2 = C::div(nL * tL, rL), FL3 = (EL + pL) * unL;
+    const R FR0 = nR, FR1 = C::div(nR * nR, rR) + pR, FR2 = C::div(nR * tR, rR), FR3 = (ER + pR) * unR;
+    // HLL with PVRS star-state speed estimates
+    const R aL = C::sqrt_(C::div(g * pL, rL));
+    const R aR = C::sqrt_(C::div(g * pR, rR));
+    const R a_ave = C::half * (aL + aR);
+    const R rho_ave = C::half * (rL + rR);
+    const R p_star = C::half * (pL + pR) - (C::half * (unR - unL)) * rho_ave * a_ave;
+    const R u_star = C::half * (unR + unL) - C::div(C::half * (pR - pL), rho_ave * a_ave);
+    const R rsL = rL + C::div((unL - u_star) * rho_ave, a_ave);
+    const R rsR = rR - C::div((unR - u_star) * rho_ave, a_ave);
+    const R asL = C::sqrt_(C::div(g * p_star, rsL));
+    const R asR = C::sqrt_(C::div(g * p_star, rsR));
+    // reference bounds: normal momentum, left state in SR; else velocities
+    const R x1 = (hll_bounds ? unL : nL) - aL, x2 = u_star - asL;
+    const R SL = (x2 < x1) ? x2 : x1;           // std::min
+    const R y1 = hll_bounds ? unR + aR : nL + aL, y2 = u_star + asR;
+    const R SR = (y1 < y2) ? y2 : y1;           // std::max
+    const R inv = C::div(R(1), SR - SL);
+    const R ss = SL * SR;
+    const R H0 = ((SR * FL0 - SL * FR0) + ss * (rR - rL)) * inv;
+    const R H1 = ((SR * FL1 - SL * FR1) + ss * (nR - nL)) * inv;
+    const R H2 = ((SR * FL2 - SL * FR2) + ss * (tR - tL)) * inv;
+    const R H3 = ((SR * FL3 - SL * FR3) + ss * (ER - EL)) * inv;
+    const bool left = SL >= R(0), right = SR <= R(0);
+    out[0] = left ? FL0 : right ? FR0 : H0;
+    out[1] = left ? FL1 : right ? FR1 : H1;
+    out[2] = left ? FL2 : right ? FR2 : H2;
+    out[3] = left ? FL3 : right ? FR3 : H3;
+}
+
+// Fetch one cell of the (halo-extended) tile, applying the physical boundary:
+// an index outside [0, n) on a Neumann / Dirichlet side is the side's single
+// ghost cell (the reference's face walk clamps to it, CellInterface.cpp:172-178):
+// Neumann ghost = copy of the adjacent interior cell (NeummanBoundary.cpp:31-45),
+// Dirichlet ghost = the fixed state (DirichletBoundary.cpp:32-46).
+__device__ __forceinline__ void fetch_cell(const StageArgs& a, int gi, int gj, double v[4]) {
+    int side = -1;
+    if (gi < 0) {
+        if (a.bc[0] == BC_PERIODIC) gi += a.nx; else side = 0;
+    } else if (gi >= a.nx) {
+        if (a.bc[1] == BC_PERIODIC) gi -= a.nx; else side = 1;
+    }
+    if (gj < 0) {
+        if (a.bc[2] == BC_PERIODIC) gj += a.ny; else if (side < 0) side = 2;
+    } else if (gj >= a.ny) {
+        if (a.bc[3] == BC_PERIODIC) gj -= a.ny; else if (side < 0) side = 3;
+    }
+    gi = gi < 0 ? 0 : (gi >= a.nx ? a.nx - 1 : gi);
+    gj = gj < 0 ? 0 : (gj >= a.ny ? a.ny - 1 : gj);
+    if (side >= 0 && a.bc[side] == BC_DIRICHLET) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = a.bcs[side][q];
+        return;
+    }
+    const long long c = (long long)gi * a.ny + gj;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = a.in[q * a.plane + c];
+}
+
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
+    __shared__ R S[SR * SC * 4];
+    __shared__ R F[(NXF + NYF) * 4];
+    const int tid = threadIdx.x;
+    const int ti = blockIdx.x / a.tiles_j;
+    const int tj = blockIdx.x - ti * a.tiles_j;
+    const int i0 = ti * TI, j0 = tj * TJ;
+    const double dt = *a.dt;
+
+    // ---- tile + halo -> LDS (corners are never read) ----
+    for (int s = tid; s < SR * SC; s += BLOCK) {
+        const int r = s / SC, c = s - r * SC;
+        const bool xs = (c >= HALO && c < HALO + TJ), ys = (r >= HALO && r < HALO + TI);
+        if (!(xs || ys)) continue;
+        double v[4];
+        fetch_cell(a, i0 - HALO + r, j0 - HALO + c, v);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) S[s * 4 + q] = (R)v[q];
+    }
+    __syncthreads();
+
+    // ---- faces ----
+    const bool px = a.bc[0] == BC_PERIODIC, py = a.bc[2] == BC_PERIODIC;
+    const R g = (R)a.gamma;
+    for (int s = tid; s < NXF + NYF; s += BLOCK) {
+        int cell[6], nrm, tng;
+        if (s < NXF) {
+            const int fr = s / TJ, jl = s - fr * TJ;
+            const int f = i0 + fr;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                int o = k - 3;
+                // boundary faces: the walk stops at the boundary face, so the
+                // stencil is [G,G,G,c0,c0,c0] at f = 0 and [c,c,c,G,G,G] at f = n
+                if (!px && f == 0 && o > 0) o = 0;
+                if (!px && f == a.nx && o < -1) o = -1;
+                cell[k] = ((fr + o + HALO) * SC + jl + HALO) * 4;
+            }
+            nrm = 1; tng = 2;
+        } else {
+            const int t = s - NXF;
+            const int il = t / (TJ + 1), gl = t - il * (TJ + 1);
+            const int gf = j0 + gl;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                int o = k - 3;
+                if (!py && gf == 0 && o > 0) o = 0;
+                if (!py && gf == a.ny && o < -1) o = -1;
+                cell[k] = ((il + HALO) * SC + gl + o + HALO) * 4;
+            }
+            nrm = 2; tng = 1;
+        }
+        R fl[4];
+        face_flux<R>(S, cell, nrm, tng, g, a.positivity, a.hll_bounds, fl);
+        F[s * 4 + 0] = fl[0];
+        F[s * 4 + nrm] = fl[1];
+        F[s * 4 + tng] = fl[2];
+        F[s * 4 + 3] = fl[3];
+    }
+    __syncthreads();
+
+    // ---- cell update: flux divergence (Cell::comRHS, Cell.cpp:110-125) and the
+    // TVD-RK combination (TVD2rdRK.cpp:3-31, TVD3rdRK.cpp:3-48) ----
+    if (tid < TI * TJ) {
+        const int il = tid / TJ, jl = tid - il * TJ;
+        const int i = i0 + il, j = j0 + jl;
+        if (i < a.nx && j < a.ny) {
+            const long long c = (long long)i * a.ny + j;
+            const bool fluid = (a.type == nullptr) || (a.type[c] == 0);
+            const R rdx = (R)a.dx, rdy = (R)a.dy;
+            const double c2dt = a.c2 * dt;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const double u = a.in[q * a.plane + c];
+                double res = u;
+                if (fluid) {
+                    const R fl = F[(il * TJ + jl) * 4 + q];
+                    const R fr = F[((il + 1) * TJ + jl) * 4 + q];
+                    const R gb = F[(NXF + il * (TJ + 1) + jl) * 4 + q];
+                    const R gt = F[(NXF + il * (TJ + 1) + jl + 1) * 4 + q];
+                    const R rhs = K<R>::div(-(fr - fl), rdx) - K<R>::div(gt - gb, rdy);
+                    const double L = (double)rhs;
+                    if (a.form == 0) res = a.c1 * u + c2dt * L;
+                    else if (a.form == 1) res = (a.c0 * a.u0[q * a.plane + c] + a.c1 * u) + c2dt * L;
+                    else res = (a.c0 * a.u0[q * a.plane + c] + a.c1 * a.u1[q * a.plane + c]) + c2dt * L;
+                }
+                a.out[q * a.plane + c] = res;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Immersed-boundary fill (Cell::comImmersedGhostPhase, Cell.cpp:172-181 ->
+// ImagePoint::InterpolatePhase, ImagePoint.cpp:20-82): each immersed-ghost
+// cell solves the donor system through its precomputed pivoted LDL^T, per
+// variable, evaluates w0*x*y + w1*x + w2*y + w3 at its image point and
+// stores (rho, -rho*u, -rho*v, E).  FP64 in every precision mode.
+struct IBArgs {
+    double* U;
+    long long plane;
+    int n;
+    int interp;           // 0: reference LDL^T solve, 1: donor weights
+    const int* cells;
+    const int* donors;
+    const double* ipxy;
+    const double* ldl;    // L10 L20 L21 L30 L31 L32 D0 D1 D2 D3
+    const int* transp;
+};
+
+__device__ __forceinline__ double pick4(const double x[4], int p) {
+    return p == 0 ? x[0] : p == 1 ? x[1] : p == 2 ? x[2] : x[3];
+}
+__device__ __forceinline__ void put4(double x[4], int p, double v) {
+    x[0] = (p == 0) ? v : x[0];
+    x[1] = (p == 1) ? v : x[1];
+    x[2] = (p == 2) ? v : x[2];
+    x[3] = (p == 3) ? v : x[3];
+}
+
+// Solve with the pivoted LDL^T of the donor matrix, in the order of Eigen's
+// LDLT::solve: transpositions, unit-lower solve (column sweep), D^-1 (zero
+// below DBL_MIN), unit-upper (L^T) solve (row dot products), transpositions
+// in reverse.
+__device__ __forceinline__ void ldlt_solve(const double* l, const int* tr, double x[4]) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int p = tr[k];
+        const double xk = x[k], xp = pick4(x, p);
+        put4(x, p, xk);
+        x[k] = xp;
+    }
+    x[1] -= x[0] * l[0];
+    x[2] -= x[0] * l[1];
+    x[3] -= x[0] * l[3];
+    x[2] -= x[1] * l[2];
+    x[3] -= x[1] * l[4];
+    x[3] -= x[2] * l[5];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const double d = l[6 + i];
+        x[i] = (__builtin_fabs(d) > 2.2250738585072014e-308) ? x[i] / d : 0.0;
+    }
+    x[2] -= l[5] * x[3];
+    x[1] -= l[2] * x[2] + l[4] * x[3];
+    x[0] -= (l[0] * x[1] + l[1] * x[2]) + l[3] * x[3];
+#pragma unroll
+    for (int k = 3; k >= 0; --k) {
+        const int p = tr[k];
+        const double xk = x[k], xp = pick4(x, p);
+        put4(x, p, xk);
+        x[k] = xp;
+    }
+}
+
+__global__ __launch_bounds__(256) void ib_fill_kernel(IBArgs a) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.n) return;
+    const int* dn = a.donors + 4 * k;
+    double f[4][4];
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        const long long c = dn[d];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) f[q][d] = a.U[q * a.plane + c];
+    }
+    const double* l = a.ldl + 10 * k;
+    double val[4];
+    if (a.interp == 0) {
+        const int* tr = a.transp + 4 * k;
+        const double x = a.ipxy[2 * k], y = a.ipxy[2 * k + 1];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            double w[4] = {f[q][0], f[q][1], f[q][2], f[q][3]};
+            ldlt_solve(l, tr, w);
+            val[q] = ((w[0] * x * y + w[1] * x) + w[2] * y) + w[3];
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) val[q] = ((l[0] * f[q][0] + l[1] * f[q][1]) + l[2] * f[q][2]) + l[3] * f[q][3];
+    }
+    const long long c = a.cells[k];
+    a.U[c] = val[0];
+    a.U[a.plane + c] = -val[1];
+    a.U[2 * a.plane + c] = -val[2];
+    a.U[3 * a.plane + c] = val[3];
+}
+
+// ---------------------------------------------------------------------------
+// CFL reduction: s = max over fluid cells of (|u|+c)/dx + (|v|+c)/dy.  Every
+// wavefront reduces with shuffles, every workgroup through LDS, and one
+// atomicMax on the bit pattern (non-negative doubles order as unsigned
+// integers; a NaN wins) combines the workgroups -- exact, so order-free.
+__device__ __forceinline__ double max_nan(double a, double b) { return (b > a || b != b) ? b : a; }
+
+__global__ __launch_bounds__(256) void cfl_kernel(const double* __restrict__ U, long long plane,
+                                                  const uint8_t* __restrict__ type, double gamma,
+                                                  double dx, double dy, int hll_bounds,
+                                                  unsigned long long* acc) {
+    double m = 0.0;
+    for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < plane;
+         c += (long long)gridDim.x * blockDim.x) {
+        if (type && type[c] != 0) continue;
+        const double rho = U[c], mu = U[plane + c], mv = U[2 * plane + c], E = U[3 * plane + c];
+        const double p = (gamma - 1) * (E - 0.5 * (mu * mu + mv * mv) / rho);
+        const double cs = __builtin_sqrt(gamma * p / rho);
+        const double u = mu / rho, v = mv / rho;
+        double ax = __builtin_fabs(u), ay = __builtin_fabs(v);
+        if (hll_bounds == 0) {   // the reference HLL's bounds use the momentum
+            if (ax < __builtin_fabs(mu)) ax = __builtin_fabs(mu);
+            if (ay < __builtin_fabs(mv)) ay = __builtin_fabs(mv);
+        }
+        const double s = (ax + cs) / dx + (ay + cs) / dy;
+        m = max_nan(m, s);
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) m = max_nan(m, __shfl_xor(m, off, 64));
+    __shared__ double wmax[4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) wmax[w] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = wmax[0];
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) b = max_nan(b, wmax[k]);
+        atomicMax(acc, (unsigned long long)__double_as_longlong(b));
+    }
+}
+
+// dt = cfl / s, optionally clipped to t_end - t; acc is cleared for the next
+// reduction (stream order makes this safe without a memset).
+struct DtArgs {
+    unsigned long long* acc;
+    double* dt;
+    double* t;          // simulated time
+    double cfl, t_end;  // t_end <= 0: no clipping
+    int* status;        // set to 1 when s is not finite or not positive
+};
+__global__ void dt_finalize_kernel(DtArgs a) {
+    const double s = __longlong_as_double((long long)*a.acc);
+    *a.acc = 0ull;
+    double dt = a.cfl / s;
+    if (!(s > 0.0) || !(dt < 1.0e300)) { *a.status = 1; dt = 0.0; }
+    if (a.t_end > 0.0) {
+        const double rem = a.t_end - *a.t;
+        if (!(dt < rem)) dt = rem > 0.0 ? rem : 0.0;
+    }
+    *a.dt = dt;
+    *a.t = *a.t + dt;
+}
+
+__global__ void set_scalar_kernel(double* p, double v) { *p = v; }
+__global__ void set_u64_kernel(unsigned long long* p, unsigned long long v) { *p = v; }
+
+// ---------------------------------------------------------------------------
+// Fixed-order checksum tree: chunks of 256 summed by halving in LDS
+// (a[t] += a[t+h], h = 128..1); the chunk sums form the next level.
+__global__ __launch_bounds__(256) void tree_sum_kernel(const double* __restrict__ src, long long n,
+                                                       long long src_stride, const uint8_t* __restrict__ type,
+                                                       double* __restrict__ dst, long long dst_stride) {
+    __shared__ double buf[256];
+    const int q = blockIdx.y;
+    const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+    double v = 0.0;
+    if (k < n) {
+        v = src[q * src_stride + k];
+        if (type && type[k] != 0) v = 0.0;
+    }
+    buf[threadIdx.x] = v;
+    __syncthreads();
+    for (int h = 128; h >= 1; h >>= 1) {
+        if ((int)threadIdx.x < h) buf[threadIdx.x] = buf[threadIdx.x] + buf[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) dst[q * dst_stride + blockIdx.x] = buf[0];
+}
+
+}  // namespace cfs

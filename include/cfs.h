@@ -1,0 +1,194 @@
+/*
+ * cfs.h -- C-ABI of the MI355X (gfx950) per-stage hot path of the 2D
+ * compressible-flow solver: WENO5 reconstruction, HLL flux, TVD-RK stage
+ * update, physical-boundary fill, ghost-cell immersed-boundary fill and the
+ * CFL time-step reduction.
+ *
+ * The reference binds this path through C++ virtual "plugins" held as global
+ * singletons and called once per cell / per face from its stage loop
+ * (Run.cpp:153-204).  That granularity cannot cross a device boundary, so this
+ * ABI sits one level up, at stage / step granularity.  Each entry point below
+ * names the reference interface whose work it takes over.
+ *
+ * Conventions
+ *   - plain C, no exceptions cross the boundary; every call returns a
+ *     cfs_status (0 = success, negative = error), cfs_status_string() names it;
+ *   - the handle owns every device buffer and one HIP stream; host arrays are
+ *     caller-owned and copied;
+ *   - a handle is not re-entrant; distinct handles may be used concurrently;
+ *   - state arrays are FP64, structure-of-arrays [4][nx][ny], x-major
+ *     (index i*ny + j), variables (rho, rho*u, rho*v, E) -- the layout the
+ *     reference writes its snapshots in (source/io/Output.cpp:60-74).
+ */
+#ifndef CFS_H
+#define CFS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CFS_ABI_VERSION 1
+
+typedef enum cfs_status {
+    CFS_OK = 0,
+    CFS_E_ARG = -1,         /* invalid argument (null pointer, bad size, bad enum) */
+    CFS_E_HIP = -2,         /* a HIP runtime call failed */
+    CFS_E_OOM = -3,         /* device allocation failed */
+    CFS_E_STATE = -4,       /* call out of sequence (e.g. stage counter mismatch) */
+    CFS_E_NONFINITE = -5,   /* a reduction met NaN/Inf (e.g. negative pressure) */
+    CFS_E_UNSUPPORTED = -6  /* configuration not supported by this build */
+} cfs_status;
+
+/* Physical boundary kinds.  NEUMANN and DIRICHLET follow
+ * source/boundary/NeummanBoundary.cpp:31-45 and DirichletBoundary.cpp:32-46
+ * (one ghost cell per boundary face); PERIODIC has no reference counterpart
+ * (the reference's Cartesian topology is non-periodic,
+ * source/parallel/CartesianParallel.cpp:7) and must be set on both sides of
+ * an axis. */
+enum { CFS_BC_NEUMANN = 0, CFS_BC_DIRICHLET = 1, CFS_BC_PERIODIC = 2 };
+/* side order of the reference's boundary vector (Reimman2D.cpp:41-51) */
+enum { CFS_SIDE_LEFT = 0, CFS_SIDE_RIGHT = 1, CFS_SIDE_BOTTOM = 2, CFS_SIDE_TOP = 3 };
+/* arithmetic of the reconstruct/flux/RHS part of a stage; the conserved state,
+ * the RK combination, the CFL reduction, checksums and the immersed-boundary
+ * fill are FP64 in both. */
+enum { CFS_PREC_FP64 = 0, CFS_PREC_FP32_BULK = 1 };
+/* Immersed-ghost interpolation from the 4 donor cells.
+ * REFERENCE: as ImagePoint::InterpolatePhase (ImagePoint.cpp:22-69) computes
+ *   it: Eigen's LDLT of the donor matrix M (rows [x*y, x, y, 1]) reads only the
+ *   lower triangle, so the system solved is sym(tril(M)) w = f, and the value
+ *   is w0*x*y + w1*x + w2*y + w3 at the image point.  This is not a bilinear
+ *   fit and can give negative densities: the reference's own cylinder case
+ *   turns non-finite within a few stages at most resolutions.
+ * BILINEAR_POSITIVE: the bilinear fit the reference intends, solved exactly
+ *   in cell-local coordinates, used when the donor system is non-degenerate and
+ *   all its weights are >= 0; otherwise inverse-squared-distance weights.  Both
+ *   branches are convex combinations (admissible states stay admissible). */
+enum { CFS_IB_REFERENCE = 0, CFS_IB_BILINEAR_POSITIVE = 1 };
+/* cell classification of source/meshes/Cell.h:14 (CellType) */
+enum { CFS_CELL_FLUID = 0, CFS_CELL_SOLID = 1, CFS_CELL_IMMERSED_GHOST = 2 };
+
+typedef struct cfs_config {
+    int32_t nx, ny;          /* cells along x and y (CartesianMesh::setnumberCells*) */
+    double lx, ly;           /* domain lengths; dx = lx/nx (CartesianMesh.cpp:150-153) */
+    double x0, y0;           /* domain origin; centre of (i,j) = ((i+0.5)*dx + x0, (j+0.5)*dy + y0) */
+    double gamma;            /* ratio of specific heats (param.h:5) */
+    int32_t bc[4];           /* CFS_BC_* per side, order L,R,B,T */
+    double bc_state[4][4];   /* Dirichlet ghost state per side, conserved (rho, rho*u, rho*v, E) */
+    int32_t rk;              /* 2 = TVD2rdRK (Run.cpp:47), 3 = TVD3rdRK as written */
+    int32_t precision;       /* CFS_PREC_* */
+    int32_t viscous;         /* 0 = Euler; 1 = Navier-Stokes terms (4th/6th-order central) */
+    int32_t positivity;      /* 0 = reference semantics.  1 = a face state whose WENO
+                                reconstruction is not admissible (rho > 0 and p > 0 fail)
+                                is replaced by the adjacent cell's state (first order);
+                                the reference turns such a face into NaN through
+                                sqrt(gamma*p/rho) whenever p/rho < 0. */
+    int32_t hll_bounds;      /* 0 = reference HLL outer wave-speed bounds
+                                SL = min(mL - aL, u* - a*L), SR = max(mL + aL, u* + a*R)
+                                with m the normal MOMENTUM and the left state in SR
+                                (RiemmanSolverHLL.cpp:41-42); 1 = velocity bounds
+                                SL = min(uL - aL, u* - a*L), SR = max(uR + aR, u* + a*R). */
+    double mu, prandtl, ma_ref; /* constant viscosity, Prandtl number, reference Mach number */
+} cfs_config;
+
+typedef struct cfs_solver cfs_solver;
+
+/* ---- library ----------------------------------------------------------- */
+int32_t     cfs_abi_version(void);
+const char* cfs_status_string(int32_t status);
+/* number of visible HIP devices (0 without a GPU; never an error) */
+int32_t     cfs_device_count(void);
+
+/* ---- solver lifetime ---------------------------------------------------- */
+/* Allocates the device state planes for one grid.  Replaces the reference's
+ * mesh/plugin construction for the hot path (Run::initialize, Run.cpp:10-90:
+ * CartesianMesh::allocateCells/allocateCellInterfaces/BindCellInterfacesWithCells,
+ * the EulerMod/Weno5th/RiemmanSolverHLL/TVD*RK singletons, the boundary
+ * objects of Initialcondition::setBounbdaryCondition).
+ * hip_stream may be NULL: the handle then creates its own stream. */
+int32_t cfs_create(const cfs_config* cfg, void* hip_stream, cfs_solver** out);
+void    cfs_destroy(cfs_solver* s);
+
+/* Host <-> device state copies ([4][nx][ny] FP64).  set_state is allowed only
+ * at a step boundary (the reference fills the initial condition before its
+ * stage loop, Run.cpp:99-105).  get_state returns the current stage's state. */
+int32_t cfs_set_state(cfs_solver* s, const double* U);
+int32_t cfs_get_state(cfs_solver* s, double* U);
+/* Same with device pointers (no host staging); the pointer must hold 4*nx*ny doubles. */
+int32_t cfs_set_state_device(cfs_solver* s, const double* dU);
+int32_t cfs_get_state_device(cfs_solver* s, double* dU);
+
+/* ---- immersed boundary --------------------------------------------------- */
+/* Host-side set-up of a circle immersed boundary, the work of
+ * CartesianMesh::setImmersedBoundary (CartesianMesh.cpp:357-374):
+ * Circle::Initialize (Circle.cpp:12-41), DistinguishWallandFluid (:43-51),
+ * SetGhostcell (:53-103), SetImagepoint (:105-138) and
+ * FindNearest4FluidCellsIndexed (:160-215), plus the factorisation that
+ * ImagePoint::InterpolatePhase (ImagePoint.cpp:20-51) repeats every stage
+ * (done once here: the geometry is fixed).
+ * interp is CFS_IB_* and selects what coef holds.
+ * Outputs (caller-allocated):
+ *   cell_type [nx*ny]         CFS_CELL_* per cell
+ *   ig_cells  [cap]           linear index i*ny+j of each immersed-ghost cell, i-major order
+ *   donors    [cap*4]         linear indices of the 4 donor fluid cells (distance order)
+ *   image_xy  [cap*2]         image-point coordinates
+ *   coef      [cap*10]        REFERENCE: pivoted LDL^T of the donor matrix, 6
+ *                             strict-lower L entries (10,20,21,30,31,32) then D[0..3];
+ *                             BILINEAR_POSITIVE: the 4 donor weights, then zeros
+ *   transp    [cap*4]         REFERENCE: pivot transpositions; otherwise 0,1,2,3
+ * *n_ig receives the number of immersed-ghost cells; CFS_E_ARG if it exceeds
+ * cap (then only *n_ig is meaningful) or if an image point has fewer than 4
+ * fluid donors in its window. Needs no GPU. */
+int32_t cfs_circle_setup(const cfs_config* cfg, double cx, double cy, double radius,
+                         int32_t n_lagrangian, int32_t n_layers, int32_t interp, int32_t cap,
+                         uint8_t* cell_type, int32_t* n_ig, int32_t* ig_cells,
+                         int32_t* donors, double* image_xy, double* coef,
+                         int32_t* transp);
+
+/* Uploads the immersed-boundary tables produced by cfs_circle_setup.  From
+ * then on every stage first fills the immersed-ghost cells from their image
+ * points (Run.cpp:156-162 -> Cell::comImmersedGhostPhase, Cell.cpp:172-181)
+ * and updates fluid cells only (Run.cpp:193-200). */
+int32_t cfs_set_immersed(cfs_solver* s, int32_t interp, const uint8_t* cell_type, int32_t n_ig,
+                         const int32_t* ig_cells, const int32_t* donors,
+                         const double* image_xy, const double* coef,
+                         const int32_t* transp);
+
+/* ---- the per-stage hot path ---------------------------------------------- */
+/* One RK stage; stage_counter is the reference's global loop counter i
+ * (Run.cpp:153, starting at 1) that TVD2rdRK/TVD3rdRK::advance key their
+ * sub-stage on (TVD2rdRK.cpp:11, TVD3rdRK.cpp:15).  It must equal the
+ * handle's next counter (CFS_E_STATE otherwise).  Replaces one iteration of
+ * Run.cpp:153-204: IB fill, (halo), WENO over all faces, HLL over all faces,
+ * RK update of fluid cells, physical-boundary fill. */
+int32_t cfs_stage(cfs_solver* s, int32_t stage_counter, double dt);
+/* All sub-stages of one RK step with time step dt (2 or 3 stages). */
+int32_t cfs_step(cfs_solver* s, double dt);
+/* nsteps RK steps.  dt > 0: fixed step.  dt <= 0: each step uses
+ * dt = cfl / s_max, s_max = max over fluid cells of
+ *   (a_x + c)/dx + (a_y + c)/dy,  c = sqrt(gamma*p/rho),
+ * a_x = max(|u|,|rho*u|) with the reference HLL bounds (hll_bounds = 0: they
+ * use the momentum where the velocity belongs, RiemmanSolverHLL.cpp:41-42),
+ * a_x = |u| with velocity bounds; likewise a_y.  Reduced on the device (no
+ * reference counterpart: the reference uses a fixed dt, param.h:20).
+ * t_end > 0 clips the last step so that the time does not pass t_end and stops there.
+ * On return *t_io is advanced by the simulated time (may be NULL). */
+int32_t cfs_advance(cfs_solver* s, int32_t nsteps, double dt, double cfl,
+                    double t_end, double* t_io, int32_t* steps_done);
+/* The CFL time step of the current state: dt = cfl / s_max (s_max as for
+ * cfs_advance).  FP64 max reduction; CFS_E_NONFINITE if s_max is not finite
+ * or not positive. */
+int32_t cfs_compute_dt(cfs_solver* s, double cfl, double* dt_out);
+/* Fixed-order FP64 tree sums of (rho, rho*u, rho*v, E) over fluid cells. */
+int32_t cfs_checksums(cfs_solver* s, double out[4]);
+/* next stage counter (1 + stages done) */
+int32_t cfs_stage_counter(const cfs_solver* s);
+/* Mean duration (ms) of the stage kernel over the last cfs_advance call,
+ * measured with HIP events on the handle's stream (0 if none). */
+int32_t cfs_last_kernel_ms(const cfs_solver* s, double* ms_per_stage);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CFS_H */
