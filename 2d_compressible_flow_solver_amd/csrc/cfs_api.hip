@@ -186,6 +186,8 @@ extern "C" {
 
 int32_t cfs_abi_version(void) { return CFS_ABI_VERSION; }
 
+int32_t cfs_config_size(void) { return (int32_t)sizeof(cfs_config); }
+
 const char* cfs_status_string(int32_t st) {
     switch (st) {
         case CFS_OK: return "CFS_OK";

@@ -96,6 +96,8 @@ typedef struct cfs_solver cfs_solver;
 
 /* ---- library ----------------------------------------------------------- */
 int32_t     cfs_abi_version(void);
+/* sizeof(cfs_config) as compiled into the library (binding layout check) */
+int32_t     cfs_config_size(void);
 const char* cfs_status_string(int32_t status);
 /* number of visible HIP devices (0 without a GPU; never an error) */
 int32_t     cfs_device_count(void);

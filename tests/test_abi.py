@@ -1,0 +1,68 @@
+"""The C-ABI library loads without a GPU, exports every entry point that
+include/cfs.h declares, and rejects bad arguments before touching HIP."""
+import ctypes
+import re
+
+import numpy as np
+import pytest
+
+import _oracle as O
+
+cfs = O.cfs
+
+
+def declared_functions():
+    src = open(cfs.HEADER_PATH).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(cfs_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_every_declared_symbol_is_exported():
+    lib = cfs.library()
+    names = declared_functions()
+    assert len(names) >= 18
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_version_and_layout():
+    lib = cfs.library()
+    assert lib.cfs_abi_version() == 1
+    lib.cfs_config_size.restype = ctypes.c_int32
+    assert lib.cfs_config_size() == ctypes.sizeof(cfs.Config)
+    for st in range(-6, 1):
+        assert lib.cfs_status_string(st)
+    assert cfs.device_count() >= 0
+
+
+def test_create_rejects_bad_configs_without_a_gpu():
+    lib = cfs.library()
+    h = ctypes.c_void_p()
+    bad = [
+        cfs.make_config(2, 10, 1.0, 1.0),                                  # too small
+        cfs.make_config(10, 10, 0.0, 1.0),                                 # zero length
+        cfs.make_config(10, 10, 1.0, 1.0, rk=4),                           # no such RK
+        cfs.make_config(10, 10, 1.0, 1.0, bc=(2, 0, 0, 0)),                # periodic on one side only
+        cfs.make_config(10, 10, 1.0, 1.0, precision=7),
+    ]
+    for c in bad:
+        assert lib.cfs_create(ctypes.byref(c), None, ctypes.byref(h)) == cfs.CFS_E_ARG
+        assert not h.value
+
+
+def test_circle_setup_argument_errors():
+    cfg = cfs.config_cylinder(64, 32, 4.0, 2.0)
+    with pytest.raises(cfs.CfsError):
+        cfs.circle_setup(cfg, 1.0, 1.0, -0.1)
+    with pytest.raises(cfs.CfsError):
+        cfs.circle_setup(cfg, 1.0, 1.0, 0.25, interp=5)
+    # capacity too small reports the count and an error
+    lib = cfs.library()
+    n = ctypes.c_int32(0)
+    ct = np.zeros(64 * 32, np.uint8)
+    one_i = np.zeros(4, np.int32)
+    one_d = np.zeros(10)
+    rc = lib.cfs_circle_setup(ctypes.byref(cfg), 1.0, 1.0, 0.25, 400, 3, 1, 1, ct.ctypes.data, ctypes.byref(n),
+                              one_i.ctypes.data, one_i.ctypes.data, one_d.ctypes.data, one_d.ctypes.data,
+                              one_i.ctypes.data)
+    assert rc == cfs.CFS_E_ARG and n.value > 1

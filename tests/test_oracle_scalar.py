@@ -1,0 +1,169 @@
+"""Pins the oracle's scalar building blocks by facts the mathematics fixes,
+hand-derived known answers and the reference's documented quirks -- not by
+re-typing its formulas.
+
+  weno5           source/interpolation/Weno5th.cpp:4-28
+  Euler flux      source/model/EulerMod.cpp:33-54
+  HLL             source/flux/RiemmanSolverHLL.cpp:5-75
+  face walk       source/meshes/CellInterface.cpp:135-195
+  RK stages       source/timeadvance/TVD2rdRK.cpp:3-31, TVD3rdRK.cpp:3-48
+  checksum tree   (no reference counterpart)
+"""
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import _oracle as O
+
+G = 1.4
+
+
+# ---- WENO5 ------------------------------------------------------------------
+def test_weno_constant():
+    for c in (1.0, -3.7, 1e-3, 12345.678):
+        r = O.weno5([c] * 5)
+        assert abs(r - c) <= 4 * np.spacing(abs(c)), (c, r)
+
+
+@pytest.mark.parametrize("a,b,c", [(1.0, 0.0, 0.0), (0.3, -1.2, 2.0), (-2.0, 0.5, 1.0)])
+def test_weno_reproduces_quadratics(a, b, c):
+    # cell averages of p(x) = a x^2 + b x + c over unit cells centred at 0..4;
+    # every candidate stencil is exact for quadratics, so whatever the nonlinear
+    # weights are, the value at the face x = 2.5 is p(2.5)
+    avg = [a * (k * k + 1.0 / 12.0) + b * k + c for k in range(5)]
+    r = O.weno5(avg)
+    exact = a * 2.5 ** 2 + b * 2.5 + c
+    assert abs(r - exact) <= 1e-13 * max(1.0, abs(exact))
+
+
+def test_weno_known_answer_pins_eps():
+    # v = (0,0,0,0,1): beta0 = beta1 = 0, beta2 = 13/12 + 1/4 = 4/3;
+    # alpha = (0.1/eps^2, 0.6/eps^2, 0.3/(eps+4/3)^2) with eps = 1/100;
+    # q2 = -1/6, q0 = q1 = 0  =>  value = -w2/6 = -9/2273780 exactly
+    exact = Fraction(-9, 2273780)
+    r = O.weno5([0.0, 0.0, 0.0, 0.0, 1.0])
+    assert abs(Fraction(r) - exact) <= abs(exact) * Fraction(1, 10 ** 14)
+    # with the textbook eps = 1e-6 the value would be ~1e-18: eps = 1e-2 is pinned
+
+
+def test_weno_non_oscillatory_at_a_step():
+    # upwind side smooth: the face value stays next to the upwind plateau
+    r = O.weno5([0.0, 0.0, 0.0, 1.0, 1.0])
+    assert 0.0 <= r < 1e-3
+    r = O.weno5([1.0, 1.0, 1.0, 0.0, 0.0])
+    assert 1.0 - 1e-3 < r <= 1.0
+
+
+# ---- Euler flux and HLL ------------------------------------------------------
+def test_euler_flux_known_answer():
+    S = [2.0, 1.0, 0.5, 5.0]           # rho, rho u, rho v, E
+    p = 0.4 * (5.0 - 0.5 * (1.0 + 0.25) / 2.0)   # 1.875
+    fx = O.euler_flux(G, S, True)
+    fy = O.euler_flux(G, S, False)
+    assert np.allclose(fx, [1.0, 0.5 + p, 0.25, (5.0 + p) * 0.5], rtol=1e-15)
+    assert np.allclose(fy, [0.5, 0.25, 0.125 + p, (5.0 + p) * 0.25], rtol=1e-15)
+    assert O.pressure(G, S) == pytest.approx(p, rel=1e-15)
+
+
+def _cons(rho, u, v, p):
+    return [rho, rho * u, rho * v, p / (G - 1) + 0.5 * (rho * u * rho * u + rho * v * rho * v) / rho]
+
+
+def test_hll_consistency():
+    # identical states: the HLL average collapses to the physical flux
+    for st in (_cons(0.7, 0.3, -0.2, 1.1), _cons(1.3, -0.4, 0.9, 0.6)):
+        for xdir in (True, False):
+            f = O.hll(G, st, st, xdir)
+            assert np.allclose(f, O.euler_flux(G, st, xdir), rtol=1e-14, atol=1e-14)
+
+
+def test_hll_supersonic_upwind_is_exact():
+    # rho = 1: momentum == velocity, u - c > 0 -> left flux, bit for bit
+    L = _cons(1.0, 3.0, 0.1, 1.0)
+    R = _cons(1.0, 2.5, 0.0, 0.8)
+    assert np.array_equal(O.hll(G, L, R, True), O.euler_flux(G, L, True))
+
+
+def test_hll_uses_momentum_in_outer_bounds():
+    # Same velocities, density 0.2: with velocities SL = u - c > 0 (upwind
+    # flux); the reference bounds use the momentum 0.2*u, so SL < 0 and the
+    # HLL average is taken instead.
+    rho = 0.2
+    L = _cons(rho, 3.0, 0.0, 0.2 / 1.4)    # c = 1
+    R = _cons(rho, 3.0, 0.0, 0.2 / 1.4)
+    u, c, m = 3.0, 1.0, rho * 3.0
+    assert u - c > 0 and m - c < 0
+    R2 = _cons(rho, 3.2, 0.0, 0.2 / 1.4)
+    f = O.hll(G, L, R2, True)
+    assert not np.array_equal(f, O.euler_flux(G, L, True))
+    # with rho = 1 the same velocity field is upwinded exactly
+    L1, R1 = _cons(1.0, 3.0, 0.0, 1.0 / 1.4), _cons(1.0, 3.2, 0.0, 1.0 / 1.4)
+    assert np.array_equal(O.hll(G, L1, R1, True), O.euler_flux(G, L1, True))
+    del R
+
+
+def test_hll_x_y_mirror():
+    # the y flux of a state equals the x flux of the state with its momentum
+    # components swapped, components swapped back (bit for bit)
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        a = _cons(rng.uniform(0.2, 2), rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(0.2, 2))
+        b = _cons(rng.uniform(0.2, 2), rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(0.2, 2))
+        fy = O.hll(G, a, b, False)
+        sa = [a[0], a[2], a[1], a[3]]
+        sb = [b[0], b[2], b[1], b[3]]
+        fx = O.hll(G, sa, sb, True)
+        assert np.array_equal(fy, [fx[0], fx[2], fx[1], fx[3]])
+
+
+# ---- face walk ----------------------------------------------------------------
+def test_face_stencils_n8():
+    # cells 0..7, ghost -1 and 8; the walk stops at boundary faces 0 and 8
+    table = {
+        0: [-1, -1, -1, 0, 0, 0],
+        1: [-1, -1, 0, 1, 2, 3],
+        2: [-1, 0, 1, 2, 3, 4],
+        3: [0, 1, 2, 3, 4, 5],
+        5: [2, 3, 4, 5, 6, 7],
+        6: [3, 4, 5, 6, 7, 8],
+        7: [4, 5, 6, 7, 8, 8],
+        8: [7, 7, 7, 8, 8, 8],
+    }
+    for f, want in table.items():
+        assert O.collect_stencil(f, 8) == want, f
+    assert O.collect_stencil(0, 8, periodic=True) == [5, 6, 7, 0, 1, 2]
+    assert O.collect_stencil(8, 8, periodic=True) == [5, 6, 7, 0, 1, 2]
+
+
+# ---- RK stage formulas --------------------------------------------------------
+def test_rk2_constant_rhs():
+    assert O.rk_apply(2, 1.0, [3.0, 3.0], 0.25) == 1.75
+
+
+def test_rk3_as_written_constant_rhs_gives_four_thirds():
+    # TVD3rdRK keeps the stage-1 snapshot in its last stage:
+    # U^{n+1} = U0 + (4/3) dt L for constant L (a consistent RK3 gives U0 + dt L)
+    r = O.rk_apply(3, 1.0, [3.0, 3.0, 3.0], 0.25)
+    assert r == pytest.approx(1.0 + 4.0 / 3.0 * 0.25 * 3.0, rel=1e-15)
+
+
+def test_rk3_as_written_ignores_second_stage_rhs():
+    # U3 = U0/3 + 2/3 (U0 + dt L1) + 2/3 dt L3: the second stage's L2 only
+    # reaches the result through the state it produces, never directly
+    a = O.rk_apply(3, 1.0, [0.5, -7.0, 2.0], 0.1)
+    b = O.rk_apply(3, 1.0, [0.5, 11.0, 2.0], 0.1)
+    assert a == b
+    assert a == pytest.approx(1.0 + 2.0 / 3.0 * 0.1 * 0.5 + 2.0 / 3.0 * 0.1 * 2.0, rel=1e-15)
+
+
+# ---- checksum tree -----------------------------------------------------------
+def test_tree_sum_order():
+    # the halving tree adds a0 + a2 first: (1e16 + -1e16) + 1 = 1, while a
+    # left-to-right sum would give 0
+    assert O.tree_sum([1e16, 1.0, -1e16]) == 1.0
+    rng = np.random.default_rng(3)
+    a = rng.normal(size=100_003)
+    assert abs(O.tree_sum(a) - math.fsum(a)) <= 1e-12 * np.abs(a).sum()
+    assert O.tree_sum([]) == 0.0
