@@ -7,6 +7,7 @@
 // point launches HIP kernels and fails with CFS_E_HIP when HIP does.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -37,6 +38,8 @@ struct cfs_solver {
     double* scal = nullptr;
     double* tree = nullptr;       // checksum scratch
     long long tree_len = 0;
+    int* redo = nullptr;          // [0..1] per-parity redo counters, [2..] redo tile list
+    int parity = 0;
     int counter = 1;
     int tiles_i = 0, tiles_j = 0;
     double last_kernel_ms = 0.0;
@@ -111,9 +114,17 @@ int launch_stage(cfs_solver* s, int counter, bool timed) {
     a.tiles_j = s->tiles_j;
     a.dx = c.lx / c.nx;
     a.dy = c.ly / c.ny;
+    a.inv_dx = 1.0 / a.dx;
+    a.inv_dy = 1.0 / a.dy;
+    int e;
+    a.dx_pow2 = std::frexp(a.dx, &e) == 0.5;   // then 1/dx is exact and x/dx == x*(1/dx)
+    a.dy_pow2 = std::frexp(a.dy, &e) == 0.5;
     a.gamma = c.gamma;
     a.positivity = c.positivity;
     a.hll_bounds = c.hll_bounds;
+    a.redo_count = s->redo + s->parity;
+    a.redo_other = s->redo + (1 - s->parity);
+    a.redo_list = s->redo + 2;
     for (int k = 0; k < 4; ++k) {
         a.bc[k] = c.bc[k];
         for (int q = 0; q < 4; ++q) a.bcs[k][q] = c.bc_state[k][q];
@@ -144,6 +155,12 @@ int launch_stage(cfs_solver* s, int counter, bool timed) {
         hipLaunchKernelGGL(stage_kernel<float>, grid, dim3(BLOCK), 0, s->stream, a);
     HIP_OK(hipGetLastError());
     if (timed) HIP_OK(hipEventRecord(s->ev.back(), s->stream));
+    if (c.precision == CFS_PREC_FP64) {
+        // tiles outside the constant-division shortcut's range (normally none)
+        hipLaunchKernelGGL(stage_redo_kernel, dim3(64), dim3(BLOCK), 0, s->stream, a);
+        HIP_OK(hipGetLastError());
+        s->parity ^= 1;
+    }
     s->cur = next;
     return CFS_OK;
 }
@@ -228,6 +245,12 @@ int32_t cfs_create(const cfs_config* cfg, void* hip_stream, cfs_solver** out) {
     if (hipMalloc(&s->scal, sizeof(double) * 8) != hipSuccess) { cfs_destroy(s); return CFS_E_OOM; }
     s->tree_len = 4 * ((s->plane + 255) / 256 + 1);
     if (hipMalloc(&s->tree, sizeof(double) * 2 * s->tree_len) != hipSuccess) { cfs_destroy(s); return CFS_E_OOM; }
+    if (hipMalloc(&s->redo, sizeof(int) * (2 + (size_t)s->tiles_i * s->tiles_j)) != hipSuccess) {
+        cfs_destroy(s);
+        return CFS_E_OOM;
+    }
+    hipLaunchKernelGGL(set_u64_kernel, dim3(1), dim3(1), 0, s->stream, reinterpret_cast<unsigned long long*>(s->redo),
+                       0ull);
     hipLaunchKernelGGL(set_scalar_kernel, dim3(1), dim3(1), 0, s->stream, dt_ptr(s), 0.0);
     hipLaunchKernelGGL(set_scalar_kernel, dim3(1), dim3(1), 0, s->stream, t_ptr(s), 0.0);
     hipLaunchKernelGGL(set_u64_kernel, dim3(1), dim3(1), 0, s->stream, acc_ptr(s), 0ull);
@@ -247,6 +270,7 @@ void cfs_destroy(cfs_solver* s) {
     hipFree(s->planes);
     hipFree(s->scal);
     hipFree(s->tree);
+    hipFree(s->redo);
     if (s->own_stream && s->stream) hipStreamDestroy(s->stream);
     delete s;
 }
@@ -472,6 +496,29 @@ int32_t cfs_checksums(cfs_solver* s, double out[4]) {
     HIP_OK(hipMemcpyAsync(tmp, src, sizeof(double) * 4, hipMemcpyDeviceToHost, s->stream));
     HIP_OK(hipStreamSynchronize(s->stream));
     for (int q = 0; q < 4; ++q) out[q] = tmp[q];
+    return CFS_OK;
+}
+
+int32_t cfs_calibrate_copy(cfs_solver* s, int32_t repeats) {
+    if (!s || repeats < 0) return CFS_E_ARG;
+    if (rk_substage(s->cfg.rk, s->counter) != 0) return CFS_E_STATE;   // spare sets hold RK stages mid-step
+    const double2* src = reinterpret_cast<const double2*>(plane_set(s, s->cur));
+    double2* dst = reinterpret_cast<double2*>(plane_set(s, (s->cur + 1) % 3));
+    const long long n = 2 * s->plane;   // 4*plane doubles = 2*plane double2
+    for (int k = 0; k < repeats; ++k) {
+        hipLaunchKernelGGL(copy_kernel, dim3(4096), dim3(256), 0, s->stream, src, dst, n);
+        HIP_OK(hipGetLastError());
+    }
+    HIP_OK(hipStreamSynchronize(s->stream));
+    return CFS_OK;
+}
+
+int32_t cfs_last_redo_tiles(cfs_solver* s, int32_t* n) {
+    if (!s || !n) return CFS_E_ARG;
+    int v = 0;   // the counter of the last stage: its parity was toggled after launch
+    HIP_OK(hipMemcpyAsync(&v, s->redo + (1 - s->parity), sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    *n = v;
     return CFS_OK;
 }
 

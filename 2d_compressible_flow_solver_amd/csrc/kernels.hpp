@@ -54,7 +54,11 @@ struct StageArgs {
     int form;             // 0: c1*in + (c2*dt)L   1: (c0*u0 + c1*in) + (c2*dt)L   2: (c0*u0 + c1*u1) + (c2*dt)L
     int positivity;       // inadmissible WENO face state -> adjacent cell state
     int hll_bounds;       // 0: reference (momentum, left state in SR), 1: velocity bounds
-    double dx, dy, gamma;
+    int dx_pow2, dy_pow2; // dx (dy) is a power of two: divide by multiplying with 1/dx
+    double dx, dy, inv_dx, inv_dy, gamma;
+    int* redo_count;      // tiles appended by this stage's FP64 pass
+    int* redo_other;      // the other stage parity's counter (cleared by the redo pass)
+    int* redo_list;
     double c0, c1, c2;
     int bc[4];
     double bcs[4][4];
@@ -69,6 +73,28 @@ template <> struct K<double> {
     static constexpr double d0 = 0.1, d1 = 0.6, d2 = 0.3, half = 0.5;
     __device__ static double div(double a, double b) { return a / b; }
     __device__ static double sqrt_(double a) { return __builtin_sqrt(a); }
+    // x/3 and x/6 without the division sequence, bit-identical to IEEE x/3,
+    // x/6 for x == 0 and 2^-900 <= |x| < 2^1020: with z = RN(1/d), q0 = x*z
+    // is within 1 ulp of x/d, so r = x - d*q0 is exact (a small multiple of
+    // ulp(q0)); q0 + r*z then differs from x/d by at most |x/d|*2^-105, while
+    // x/d (d = 3*2^k) lies at least 2^(e-53)/3 from every rounding midpoint of
+    // its binade 2^e, so the single rounding of the fma gives RN(x/d).  q0 is
+    // returned for zero to keep the sign.  The stage kernel takes this path only
+    // for tiles whose staged values all lie in that range.
+    __device__ static double div3c(double x) {
+        const double z = 0x1.5555555555555p-2;
+        const double q0 = x * z;
+        const double r = __builtin_fma(-q0, 3.0, x);
+        const double qq = __builtin_fma(r, z, q0);
+        return q0 == 0.0 ? q0 : qq;
+    }
+    __device__ static double div6c(double x) {
+        const double z = 0x1.5555555555555p-3;
+        const double q0 = x * z;
+        const double r = __builtin_fma(-q0, 6.0, x);
+        const double qq = __builtin_fma(r, z, q0);
+        return q0 == 0.0 ? q0 : qq;
+    }
 };
 template <> struct K<float> {
     static constexpr float eps = 1e-2f, c1312 = 13.0f / 12.0f, q = 0.25f;
@@ -76,14 +102,30 @@ template <> struct K<float> {
     // FP32-bulk: hardware reciprocal (1 ulp) instead of the IEEE division
     __device__ static float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
     __device__ static float sqrt_(float a) { return __builtin_amdgcn_sqrtf(a); }
+    __device__ static float div3c(float x) { return x * (1.0f / 3.0f); }
+    __device__ static float div6c(float x) { return x * (1.0f / 6.0f); }
 };
+
+// division by the constants 3 and 6 of the WENO5 candidate polynomials
+template <typename R, bool FAST> struct CDiv {
+    __device__ static R d3(R x) { return FAST ? K<R>::div3c(x) : K<R>::div(x, R(3)); }
+    __device__ static R d6(R x) { return FAST ? K<R>::div6c(x) : K<R>::div(x, R(6)); }
+};
+
+// range in which div3c/div6c are exact for every staged value and its
+// multiples by 5, 7 and 11
+__device__ __forceinline__ bool fast_div_safe(double v) {
+    const double a = __builtin_fabs(v);
+    return v == 0.0 || (a >= 0x1p-900 && a <= 0x1p+1000);
+}
 
 // WENO5 value at the face from five cell values, v2 the upwind cell
 // (source/interpolation/Weno5th.cpp:4-28: Jiang-Shu smoothness indicators,
 // eps = 1e-2, ideal weights 0.1/0.6/0.3).  Operation order as the reference.
-template <typename R>
+template <typename R, bool FAST = false>
 __device__ __forceinline__ R weno5(R v0, R v1, R v2, R v3, R v4) {
     using C = K<R>;
+    using D = CDiv<R, FAST>;
     const R s0 = (v0 - R(2) * v1) + v2;
     const R t0 = (v0 - R(4) * v1) + R(3) * v2;
     const R b0 = C::c1312 * (s0 * s0) + C::q * (t0 * t0);
@@ -99,9 +141,9 @@ __device__ __forceinline__ R weno5(R v0, R v1, R v2, R v3, R v4) {
     const R a2 = C::div(C::d2, e2 * e2);
     const R sum = (a0 + a1) + a2;
     const R w0 = C::div(a0, sum), w1 = C::div(a1, sum), w2 = C::div(a2, sum);
-    const R q0 = (C::div(v0, R(3)) - C::div(R(7) * v1, R(6))) + C::div(R(11) * v2, R(6));
-    const R q1 = (C::div(-v1, R(6)) + C::div(R(5) * v2, R(6))) + C::div(v3, R(3));
-    const R q2 = (C::div(v2, R(3)) + C::div(R(5) * v3, R(6))) - C::div(v4, R(6));
+    const R q0 = (D::d3(v0) - D::d6(R(7) * v1)) + D::d6(R(11) * v2);
+    const R q1 = (D::d6(-v1) + D::d6(R(5) * v2)) + D::d3(v3);
+    const R q2 = (D::d3(v2) + D::d6(R(5) * v3)) - D::d6(v4);
     return (w0 * q0 + w1 * q1) + w2 * q2;
 }
 
@@ -113,7 +155,7 @@ __device__ __forceinline__ R weno5(R v0, R v1, R v2, R v3, R v4) {
 // The x and y forms of the reference map onto this one form because IEEE
 // addition and multiplication are commutative (mx*my == my*mx bit for bit).
 // out = (mass, normal momentum, tangential momentum, energy) flux.
-template <typename R>
+template <typename R, bool FAST>
 __device__ __forceinline__ void face_flux(const R* __restrict__ S, const int cell[6], int nrm, int tng,
                                           R g, int positivity, int hll_bounds, R out[4]) {
     using C = K<R>;
@@ -124,8 +166,8 @@ __device__ __forceinline__ void face_flux(const R* __restrict__ S, const int cel
         R v[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) v[k] = S[cell[k] + comp[q]];
-        L[q] = weno5<R>(v[0], v[1], v[2], v[3], v[4]);
-        Rr[q] = weno5<R>(v[5], v[4], v[3], v[2], v[1]);
+        L[q] = weno5<R, FAST>(v[0], v[1], v[2], v[3], v[4]);
+        Rr[q] = weno5<R, FAST>(v[5], v[4], v[3], v[2], v[1]);
         cl[q] = v[2];
         cr[q] = v[3];
     }
@@ -207,15 +249,19 @@ __device__ __forceinline__ void fetch_cell(const StageArgs& a, int gi, int gj, d
     for (int q = 0; q < 4; ++q) v[q] = a.in[q * a.plane + c];
 }
 
-template <typename R>
-__global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
-    __shared__ R S[SR * SC * 4];
-    __shared__ R F[(NXF + NYF) * 4];
+// One tile: stage it (with halo) in LDS, compute its 511 faces, update its
+// cells.  FAST selects the exact constant-division shortcut for the WENO5
+// candidate polynomials; a FAST pass gives up on a tile holding a value outside
+// the shortcut's range (returns false before writing anything) so that the
+// IEEE-division pass can redo it.
+template <typename R, bool FAST>
+__device__ __forceinline__ bool stage_tile(const StageArgs& a, int tile, R* S, R* F) {
     const int tid = threadIdx.x;
-    const int ti = blockIdx.x / a.tiles_j;
-    const int tj = blockIdx.x - ti * a.tiles_j;
+    const int ti = tile / a.tiles_j;
+    const int tj = tile - ti * a.tiles_j;
     const int i0 = ti * TI, j0 = tj * TJ;
     const double dt = *a.dt;
+    int unsafe = 0;
 
     // ---- tile + halo -> LDS (corners are never read) ----
     for (int s = tid; s < SR * SC; s += BLOCK) {
@@ -225,9 +271,16 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
         double v[4];
         fetch_cell(a, i0 - HALO + r, j0 - HALO + c, v);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) S[s * 4 + q] = (R)v[q];
+        for (int q = 0; q < 4; ++q) {
+            S[s * 4 + q] = (R)v[q];
+            if (FAST && sizeof(R) == 8) unsafe |= !fast_div_safe(v[q]);
+        }
     }
-    __syncthreads();
+    if (FAST && sizeof(R) == 8) {
+        if (__syncthreads_or(unsafe)) return false;   // uniform: every thread sees the OR
+    } else {
+        __syncthreads();
+    }
 
     // ---- faces ----
     const bool px = a.bc[0] == BC_PERIODIC, py = a.bc[2] == BC_PERIODIC;
@@ -261,7 +314,7 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
             nrm = 2; tng = 1;
         }
         R fl[4];
-        face_flux<R>(S, cell, nrm, tng, g, a.positivity, a.hll_bounds, fl);
+        face_flux<R, FAST>(S, cell, nrm, tng, g, a.positivity, a.hll_bounds, fl);
         F[s * 4 + 0] = fl[0];
         F[s * 4 + nrm] = fl[1];
         F[s * 4 + tng] = fl[2];
@@ -278,17 +331,23 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
             const long long c = (long long)i * a.ny + j;
             const bool fluid = (a.type == nullptr) || (a.type[c] == 0);
             const R rdx = (R)a.dx, rdy = (R)a.dy;
+            const R idx = (R)a.inv_dx, idy = (R)a.inv_dy;
             const double c2dt = a.c2 * dt;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const double u = a.in[q * a.plane + c];
+                // the FP64 tile holds the exact state; FP32-bulk re-reads it
+                const double u = (sizeof(R) == 8) ? (double)S[((il + HALO) * SC + jl + HALO) * 4 + q]
+                                                  : a.in[q * a.plane + c];
                 double res = u;
                 if (fluid) {
                     const R fl = F[(il * TJ + jl) * 4 + q];
                     const R fr = F[((il + 1) * TJ + jl) * 4 + q];
                     const R gb = F[(NXF + il * (TJ + 1) + jl) * 4 + q];
                     const R gt = F[(NXF + il * (TJ + 1) + jl + 1) * 4 + q];
-                    const R rhs = K<R>::div(-(fr - fl), rdx) - K<R>::div(gt - gb, rdy);
+                    // x/d == x*(1/d) bit for bit when d is a power of two
+                    const R tx = a.dx_pow2 ? (-(fr - fl)) * idx : K<R>::div(-(fr - fl), rdx);
+                    const R ty = a.dy_pow2 ? (gt - gb) * idy : K<R>::div(gt - gb, rdy);
+                    const R rhs = tx - ty;
                     const double L = (double)rhs;
                     if (a.form == 0) res = a.c1 * u + c2dt * L;
                     else if (a.form == 1) res = (a.c0 * a.u0[q * a.plane + c] + a.c1 * u) + c2dt * L;
@@ -297,6 +356,37 @@ __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
                 a.out[q * a.plane + c] = res;
             }
         }
+    }
+    return true;
+}
+
+// The stage kernel: one workgroup per tile.  An FP64 tile outside the
+// constant-division shortcut's range is appended to redo_list[redo_count[0]]
+// and left to stage_redo_kernel.
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
+    __shared__ R S[SR * SC * 4];
+    __shared__ R F[(NXF + NYF) * 4];
+    if (!stage_tile<R, true>(a, blockIdx.x, S, F)) {
+        if (threadIdx.x == 0) {
+            const int k = atomicAdd(a.redo_count, 1);
+            a.redo_list[k] = blockIdx.x;
+        }
+    }
+}
+
+// Redo pass with IEEE division for the tiles listed by stage_kernel (normally
+// none: every block then exits at once).  It also clears the other counter,
+// whose list the previous stage's redo pass has consumed (stream order), so
+// the next stage starts from zero without a memset.
+__global__ __launch_bounds__(BLOCK) void stage_redo_kernel(StageArgs a) {
+    __shared__ double S[SR * SC * 4];
+    __shared__ double F[(NXF + NYF) * 4];
+    const int n = *a.redo_count;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a.redo_other = 0;
+    for (int k = blockIdx.x; k < n; k += gridDim.x) {
+        stage_tile<double, false>(a, a.redo_list[k], S, F);
+        __syncthreads();
     }
 }
 
@@ -456,6 +546,14 @@ __global__ void dt_finalize_kernel(DtArgs a) {
     }
     *a.dt = dt;
     *a.t = *a.t + dt;
+}
+
+// plain streaming copy (16-byte vectors), the byte-counter calibration kernel
+__global__ __launch_bounds__(256) void copy_kernel(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                   long long n) {
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (long long)gridDim.x * blockDim.x)
+        dst[k] = src[k];
 }
 
 __global__ void set_scalar_kernel(double* p, double v) { *p = v; }

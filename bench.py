@@ -1,10 +1,17 @@
 """Benchmark of the per-RK-stage hot path on one MI355X.
 
-Workload (the headline configuration that fits one GPU): Mach 3 inviscid
-flow past a circular cylinder with the ghost-cell immersed boundary,
-2048 x 1024 cells on [0,4] x [0,2], r = 0.125 at (1,1), freestream
-rho = 1.4, u = 3, p = 1, supersonic Dirichlet inflow on the left and
-extrapolation (Neumann) elsewhere, gamma = 1.4, TVD-RK3, CFL 0.5, FP64.
+Workload (the headline configuration that fits one GPU): inviscid flow past
+a circular cylinder with the ghost-cell immersed boundary, 2048 x 1024 cells
+on [0,4] x [0,2], r = 0.125 at (1,1), 400 Lagrangian points, 3 ghost layers,
+Dirichlet inflow on the left and extrapolation (Neumann) elsewhere,
+gamma = 1.4, TVD-RK3, CFL 0.5, FP64, the reference's scheme unchanged
+(reference HLL bounds, no positivity fallback).  Inflow: the reference's own
+cylinder freestream rho = 1, u = 1, p = 1 (Cylinder.cpp:5-16): with the
+reference's scheme a Mach-3 inflow turns non-finite within the first 100
+steps at this resolution, and this inflow within ~800, so the timed window is
+520 steps (20 warm-up + 500).  The image-point interpolation is the
+positivity-preserving bilinear one (the reference's LDLT-on-lower-triangle
+interpolation gives negative ghost densities on this grid at step 1).
 One bench "step" = one RK3 step (3 stages: IB fill + fused
 WENO5/HLL/RK stage kernel each, plus the device CFL reduction).
 
@@ -37,6 +44,7 @@ HBM_PEAK_GBS = 8000.0
 NX, NY, LX, LY = 2048, 1024, 4.0, 2.0
 CX, CY, RADIUS = 1.0, 1.0, 0.125
 CFL = 0.5
+FREESTREAM = (1.0, 1.0, 0.0, 1.0)
 BYTES_PER_CELL_STAGE = 97.0
 
 
@@ -54,9 +62,10 @@ def traffic_from_profiles():
 
 
 def setup(precision):
-    cfg = cfs.config_cylinder(NX, NY, LX, LY, precision=precision)
-    ib = cfs.circle_setup(cfg, CX, CY, RADIUS, 400, 3)
-    U = cfs.cylinder_freestream(cfg, ib["cell_type"])
+    cfg = cfs.config_cylinder(NX, NY, LX, LY, freestream=FREESTREAM, precision=precision, positivity=0,
+                              hll_bounds=0)
+    ib = cfs.circle_setup(cfg, CX, CY, RADIUS, 400, 3, interp=cfs.IB_BILINEAR_POSITIVE)
+    U = cfs.cylinder_freestream(cfg, ib["cell_type"], freestream=FREESTREAM)
     return cfg, ib, U
 
 
@@ -86,7 +95,7 @@ def run_cpu_baseline(budget_s=12.0):
     cfg, ib, U = setup(cfs.PREC_FP64)
     nthreads = max(1, min(16, os.cpu_count() or 1))
     o = O.Oracle(cfg, nthreads=nthreads)
-    o.set_circle(CX, CY, RADIUS, 400, 3)
+    o.set_ib_table(ib)
     o.set_state(U)
     steps = 0
     t0 = time.perf_counter()
@@ -107,10 +116,13 @@ def run_cpu_baseline(budget_s=12.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-bulk leg")
+    ap.add_argument("--fp32-only", action="store_true", help="time the FP32-bulk leg only (profiling)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="N",
+                    help="only run N plain copy-kernel launches over one state plane-set (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -120,8 +132,23 @@ def main():
     if cfs.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.copy_calibration:
+        cfg, ib, U = setup(cfs.PREC_FP64)
+        s = cfs.Solver(cfg)
+        s.set_state(U)
+        rc = cfs.library().cfs_calibrate_copy(s._h, args.copy_calibration)
+        print(json.dumps({"copy_calibration": {"launches": args.copy_calibration, "rc": rc,
+                                               "bytes_read_per_launch": 32 * NX * NY,
+                                               "bytes_written_per_launch": 32 * NX * NY}}))
+        return
+    if args.fp32_only:
+        r = run_gpu(cfs.PREC_FP32_BULK, args.steps, args.warmup)
+        print(json.dumps({"fp32_bulk_only": r}))
+        return
 
     r64 = run_gpu(cfs.PREC_FP64, args.steps, args.warmup)
+    if not r64["finite"]:
+        print("FP64 state not finite at the end of the timed window", file=sys.stderr)
     print("fp64: %d steps, %.3f s, %.1f Mcell-updates/s, stage kernel %.4f ms"
           % (r64["steps"], r64["wall"], r64["mcups"], r64["kernel_ms"]), file=sys.stderr, flush=True)
     r32 = None
@@ -149,11 +176,13 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seedless uniform Mach-3 freestream initial state)",
-        "config": {"workload": "cylinder_mach3_ib_2048x1024_rk3_cfl0.5_fp64", "nx": NX, "ny": NY,
+        "data": "synthetic (uniform freestream rho=1, u=1, p=1 on fluid cells; no seed needed)",
+        "config": {"workload": "cylinder_ib_2048x1024_rk3_cfl0.5_fp64", "nx": NX, "ny": NY,
+                   "inflow": {"rho": 1.0, "u": 1.0, "v": 0.0, "p": 1.0},
                    "domain": [LX, LY], "cylinder": {"centre": [CX, CY], "radius": RADIUS,
                                                     "lagrangian_points": 400, "ghost_layers": 3},
-                   "rk": 3, "cfl": CFL, "precision": "fp64", "stages_per_step": 3},
+                   "rk": 3, "cfl": CFL, "precision": "fp64", "stages_per_step": 3,
+                   "ib_interpolation": "bilinear_positive", "finite_at_end": r64["finite"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
                      "kernel": "stage_kernel<double>", "kernel_ms": round(r64["kernel_ms"], 5),

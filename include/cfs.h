@@ -184,6 +184,15 @@ int32_t cfs_advance(cfs_solver* s, int32_t nsteps, double dt, double cfl,
 int32_t cfs_compute_dt(cfs_solver* s, double cfl, double* dt_out);
 /* Fixed-order FP64 tree sums of (rho, rho*u, rho*v, E) over fluid cells. */
 int32_t cfs_checksums(cfs_solver* s, double out[4]);
+/* Copies the current state plane-set into a spare one `repeats` times with a
+ * plain streaming copy kernel (32*nx*ny bytes read and written per launch).
+ * No reference counterpart: it calibrates hardware byte counters against a
+ * kernel of known traffic. */
+int32_t cfs_calibrate_copy(cfs_solver* s, int32_t repeats);
+/* Number of tiles of the last FP64 stage that were recomputed with IEEE
+ * division because they held values outside the range (|v| in [2^-900, 2^1000]
+ * or 0) where the kernel's exact division-by-3/6 shortcut applies. */
+int32_t cfs_last_redo_tiles(cfs_solver* s, int32_t* n);
 /* next stage counter (1 + stages done) */
 int32_t cfs_stage_counter(const cfs_solver* s);
 /* Mean duration (ms) of the stage kernel over the last cfs_advance call,

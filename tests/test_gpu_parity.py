@@ -6,6 +6,8 @@ reference's operations in source order with IEEE division/sqrt and no FMA
 contraction, so the tests also assert the stronger property that the states
 are bit-identical.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -175,6 +177,28 @@ def test_state_sequence_errors():
         s.step(1e-4)              # not at a step boundary
     with pytest.raises(cfs.CfsError):
         s.set_state(cfs.riemann_quadrants(cfg))
+
+
+def test_exact_division_redo_path():
+    # momentum of order 1e-300 in a patch: those tiles fall outside the range of
+    # the kernel's division-by-3/6 shortcut and are recomputed with IEEE division
+    cfg = cfs.make_config(64, 48, 1.0, 0.75, rk=3)
+    U = cfs.riemann_quadrants(cfg)
+    U[2, 20:30, 10:20] = 1e-300
+    U[1, 40:44, 30:34] = -3e-305
+    s, o = _pair(cfg, U)
+    dt = 0.4 / o.cfl_smax()
+    lib = cfs.library()
+    n = ctypes.c_int32(-1)
+    redo = []
+    for i in range(1, 13):
+        s.stage(i, dt)
+        o.stage(i, dt)
+        assert lib.cfs_last_redo_tiles(s._h, ctypes.byref(n)) == 0
+        redo.append(n.value)
+        _assert_parity(s.get_state(), o.get_state(), "redo path, stage %d" % i)
+    print("redo tiles per stage:", redo)
+    assert redo[0] > 0
 
 
 def test_fp32_bulk_l1_density():
