@@ -334,6 +334,15 @@ class Solver:
         _check(self._lib.cfs_checksums(self._h, _ptr(out)), "cfs_checksums")
         return out
 
+    def set_kernel_timing(self, on=True):
+        _check(self._lib.cfs_set_kernel_timing(self._h, 1 if on else 0), "cfs_set_kernel_timing")
+
+    def exact_tiles(self):
+        """Tiles of the last FP64 stage that took IEEE division for x/3, x/6."""
+        n = ctypes.c_int32(0)
+        _check(self._lib.cfs_last_redo_tiles(self._h, ctypes.byref(n)), "cfs_last_redo_tiles")
+        return n.value
+
     def last_kernel_ms(self):
         ms = ctypes.c_double(0.0)
         _check(self._lib.cfs_last_kernel_ms(self._h, ctypes.byref(ms)), "cfs_last_kernel_ms")

@@ -38,8 +38,9 @@ struct cfs_solver {
     double* scal = nullptr;
     double* tree = nullptr;       // checksum scratch
     long long tree_len = 0;
-    int* redo = nullptr;          // [0..1] per-parity redo counters, [2..] redo tile list
+    int* redo = nullptr;          // per-stage-parity counters of tiles that took IEEE x/3, x/6
     int parity = 0;
+    bool kernel_timing = false;   // bracket stage kernels with HIP events in cfs_advance
     int counter = 1;
     int tiles_i = 0, tiles_j = 0;
     double last_kernel_ms = 0.0;
@@ -122,9 +123,8 @@ int launch_stage(cfs_solver* s, int counter, bool timed) {
     a.gamma = c.gamma;
     a.positivity = c.positivity;
     a.hll_bounds = c.hll_bounds;
-    a.redo_count = s->redo + s->parity;
-    a.redo_other = s->redo + (1 - s->parity);
-    a.redo_list = s->redo + 2;
+    a.exact_count = s->redo + s->parity;
+    a.exact_other = s->redo + (1 - s->parity);
     for (int k = 0; k < 4; ++k) {
         a.bc[k] = c.bc[k];
         for (int q = 0; q < 4; ++q) a.bcs[k][q] = c.bc_state[k][q];
@@ -155,12 +155,7 @@ int launch_stage(cfs_solver* s, int counter, bool timed) {
         hipLaunchKernelGGL(stage_kernel<float>, grid, dim3(BLOCK), 0, s->stream, a);
     HIP_OK(hipGetLastError());
     if (timed) HIP_OK(hipEventRecord(s->ev.back(), s->stream));
-    if (c.precision == CFS_PREC_FP64) {
-        // tiles outside the constant-division shortcut's range (normally none)
-        hipLaunchKernelGGL(stage_redo_kernel, dim3(64), dim3(BLOCK), 0, s->stream, a);
-        HIP_OK(hipGetLastError());
-        s->parity ^= 1;
-    }
+    s->parity ^= 1;
     s->cur = next;
     return CFS_OK;
 }
@@ -245,7 +240,7 @@ int32_t cfs_create(const cfs_config* cfg, void* hip_stream, cfs_solver** out) {
     if (hipMalloc(&s->scal, sizeof(double) * 8) != hipSuccess) { cfs_destroy(s); return CFS_E_OOM; }
     s->tree_len = 4 * ((s->plane + 255) / 256 + 1);
     if (hipMalloc(&s->tree, sizeof(double) * 2 * s->tree_len) != hipSuccess) { cfs_destroy(s); return CFS_E_OOM; }
-    if (hipMalloc(&s->redo, sizeof(int) * (2 + (size_t)s->tiles_i * s->tiles_j)) != hipSuccess) {
+    if (hipMalloc(&s->redo, sizeof(int) * 2) != hipSuccess) {
         cfs_destroy(s);
         return CFS_E_OOM;
     }
@@ -394,7 +389,7 @@ int32_t cfs_advance(cfs_solver* s, int32_t nsteps, double dt, double cfl, double
     HIP_OK(hipGetLastError());
     for (auto e : s->ev) hipEventDestroy(e);
     s->ev.clear();
-    const bool timed = true;
+    const bool timed = s->kernel_timing;
     int rc;
     if (!use_cfl) {
         double dtx = dt;
@@ -510,6 +505,12 @@ int32_t cfs_calibrate_copy(cfs_solver* s, int32_t repeats) {
         HIP_OK(hipGetLastError());
     }
     HIP_OK(hipStreamSynchronize(s->stream));
+    return CFS_OK;
+}
+
+int32_t cfs_set_kernel_timing(cfs_solver* s, int32_t on) {
+    if (!s) return CFS_E_ARG;
+    s->kernel_timing = on != 0;
     return CFS_OK;
 }
 

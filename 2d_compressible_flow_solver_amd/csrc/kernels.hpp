@@ -56,9 +56,8 @@ struct StageArgs {
     int hll_bounds;       // 0: reference (momentum, left state in SR), 1: velocity bounds
     int dx_pow2, dy_pow2; // dx (dy) is a power of two: divide by multiplying with 1/dx
     double dx, dy, inv_dx, inv_dy, gamma;
-    int* redo_count;      // tiles appended by this stage's FP64 pass
-    int* redo_other;      // the other stage parity's counter (cleared by the redo pass)
-    int* redo_list;
+    int* exact_count;     // tiles of this stage outside the x/3, x/6 shortcut's exact range
+    int* exact_other;     // the previous stage's counter: cleared by this stage
     double c0, c1, c2;
     int bc[4];
     double bcs[4][4];
@@ -106,10 +105,15 @@ template <> struct K<float> {
     __device__ static float div6c(float x) { return x * (1.0f / 6.0f); }
 };
 
-// division by the constants 3 and 6 of the WENO5 candidate polynomials
-template <typename R, bool FAST> struct CDiv {
-    __device__ static R d3(R x) { return FAST ? K<R>::div3c(x) : K<R>::div(x, R(3)); }
-    __device__ static R d6(R x) { return FAST ? K<R>::div6c(x) : K<R>::div(x, R(6)); }
+// Division by the constants 3 and 6 of the WENO5 candidate polynomials: the
+// shortcut above, bit-identical to IEEE division for every value that is 0 or
+// lies in [2^-900, 2^1000] (and its multiples by 5, 7, 11).  Outside that range
+// -- magnitudes below 1e-270, or inf in a state that has already blown up --
+// the result can differ from IEEE division by a few subnormal ulps (absolute
+// error below 1e-320); the stage kernel counts such tiles (exact_count).
+template <typename R> struct CDiv {
+    __device__ static R d3(R x) { return K<R>::div3c(x); }
+    __device__ static R d6(R x) { return K<R>::div6c(x); }
 };
 
 // range in which div3c/div6c are exact for every staged value and its
@@ -122,10 +126,10 @@ __device__ __forceinline__ bool fast_div_safe(double v) {
 // WENO5 value at the face from five cell values, v2 the upwind cell
 // (source/interpolation/Weno5th.cpp:4-28: Jiang-Shu smoothness indicators,
 // eps = 1e-2, ideal weights 0.1/0.6/0.3).  Operation order as the reference.
-template <typename R, bool FAST = false>
+template <typename R>
 __device__ __forceinline__ R weno5(R v0, R v1, R v2, R v3, R v4) {
     using C = K<R>;
-    using D = CDiv<R, FAST>;
+    using D = CDiv<R>;
     const R s0 = (v0 - R(2) * v1) + v2;
     const R t0 = (v0 - R(4) * v1) + R(3) * v2;
     const R b0 = C::c1312 * (s0 * s0) + C::q * (t0 * t0);
@@ -155,7 +159,7 @@ __device__ __forceinline__ R weno5(R v0, R v1, R v2, R v3, R v4) {
 // The x and y forms of the reference map onto this one form because IEEE
 // addition and multiplication are commutative (mx*my == my*mx bit for bit).
 // out = (mass, normal momentum, tangential momentum, energy) flux.
-template <typename R, bool FAST>
+template <typename R>
 __device__ __forceinline__ void face_flux(const R* __restrict__ S, const int cell[6], int nrm, int tng,
                                           R g, int positivity, int hll_bounds, R out[4]) {
     using C = K<R>;
@@ -166,8 +170,8 @@ __device__ __forceinline__ void face_flux(const R* __restrict__ S, const int cel
         R v[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) v[k] = S[cell[k] + comp[q]];
-        L[q] = weno5<R, FAST>(v[0], v[1], v[2], v[3], v[4]);
-        Rr[q] = weno5<R, FAST>(v[5], v[4], v[3], v[2], v[1]);
+        L[q] = weno5<R>(v[0], v[1], v[2], v[3], v[4]);
+        Rr[q] = weno5<R>(v[5], v[4], v[3], v[2], v[1]);
         cl[q] = v[2];
         cr[q] = v[3];
     }
@@ -250,18 +254,17 @@ __device__ __forceinline__ void fetch_cell(const StageArgs& a, int gi, int gj, d
 }
 
 // One tile: stage it (with halo) in LDS, compute its 511 faces, update its
-// cells.  FAST selects the exact constant-division shortcut for the WENO5
-// candidate polynomials; a FAST pass gives up on a tile holding a value outside
-// the shortcut's range (returns false before writing anything) so that the
-// IEEE-division pass can redo it.
-template <typename R, bool FAST>
-__device__ __forceinline__ bool stage_tile(const StageArgs& a, int tile, R* S, R* F) {
+// cells.  Tiles holding a value outside the range where the WENO5 constant
+// divisions are bit-identical to IEEE division are counted in exact_count.
+template <typename R>
+__device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R* F) {
     const int tid = threadIdx.x;
     const int ti = tile / a.tiles_j;
     const int tj = tile - ti * a.tiles_j;
     const int i0 = ti * TI, j0 = tj * TJ;
     const double dt = *a.dt;
     int unsafe = 0;
+    if (tile == 0 && tid == 0) *a.exact_other = 0;   // the previous stage's counter, consumed
 
     // ---- tile + halo -> LDS (corners are never read) ----
     for (int s = tid; s < SR * SC; s += BLOCK) {
@@ -273,14 +276,12 @@ __device__ __forceinline__ bool stage_tile(const StageArgs& a, int tile, R* S, R
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             S[s * 4 + q] = (R)v[q];
-            if (FAST && sizeof(R) == 8) unsafe |= !fast_div_safe(v[q]);
+            if (sizeof(R) == 8) unsafe |= !fast_div_safe(v[q]);
         }
     }
-    if (FAST && sizeof(R) == 8) {
-        if (__syncthreads_or(unsafe)) return false;   // uniform: every thread sees the OR
-    } else {
-        __syncthreads();
-    }
+    // one barrier + OR over the tile: count tiles outside the range where the
+    // constant-division shortcut is bit-identical to IEEE division
+    if (__syncthreads_or(unsafe) && tid == 0) atomicAdd(a.exact_count, 1);
 
     // ---- faces ----
     const bool px = a.bc[0] == BC_PERIODIC, py = a.bc[2] == BC_PERIODIC;
@@ -314,7 +315,7 @@ __device__ __forceinline__ bool stage_tile(const StageArgs& a, int tile, R* S, R
             nrm = 2; tng = 1;
         }
         R fl[4];
-        face_flux<R, FAST>(S, cell, nrm, tng, g, a.positivity, a.hll_bounds, fl);
+        face_flux<R>(S, cell, nrm, tng, g, a.positivity, a.hll_bounds, fl);
         F[s * 4 + 0] = fl[0];
         F[s * 4 + nrm] = fl[1];
         F[s * 4 + tng] = fl[2];
@@ -357,37 +358,14 @@ __device__ __forceinline__ bool stage_tile(const StageArgs& a, int tile, R* S, R
             }
         }
     }
-    return true;
 }
 
-// The stage kernel: one workgroup per tile.  An FP64 tile outside the
-// constant-division shortcut's range is appended to redo_list[redo_count[0]]
-// and left to stage_redo_kernel.
+// The stage kernel: one workgroup per tile.
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
     __shared__ R S[SR * SC * 4];
     __shared__ R F[(NXF + NYF) * 4];
-    if (!stage_tile<R, true>(a, blockIdx.x, S, F)) {
-        if (threadIdx.x == 0) {
-            const int k = atomicAdd(a.redo_count, 1);
-            a.redo_list[k] = blockIdx.x;
-        }
-    }
-}
-
-// Redo pass with IEEE division for the tiles listed by stage_kernel (normally
-// none: every block then exits at once).  It also clears the other counter,
-// whose list the previous stage's redo pass has consumed (stream order), so
-// the next stage starts from zero without a memset.
-__global__ __launch_bounds__(BLOCK) void stage_redo_kernel(StageArgs a) {
-    __shared__ double S[SR * SC * 4];
-    __shared__ double F[(NXF + NYF) * 4];
-    const int n = *a.redo_count;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *a.redo_other = 0;
-    for (int k = blockIdx.x; k < n; k += gridDim.x) {
-        stage_tile<double, false>(a, a.redo_list[k], S, F);
-        __syncthreads();
-    }
+    stage_tile<R>(a, blockIdx.x, S, F);
 }
 
 // ---------------------------------------------------------------------------

@@ -79,6 +79,10 @@ def run_gpu(precision, steps, warmup):
     t0 = time.perf_counter()
     t, done = s.advance(steps, dt=0.0, cfl=CFL)
     wall = time.perf_counter() - t0
+    # stage-kernel duration: a separate pass with HIP events around every
+    # stage kernel (the events add queue latency, so not in the timed window)
+    s.set_kernel_timing(True)
+    s.advance(min(steps, 40), dt=0.0, cfl=CFL)
     kms = s.last_kernel_ms()
     V = s.get_state()
     ok = bool(np.all(np.isfinite(V)))

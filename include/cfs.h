@@ -189,14 +189,20 @@ int32_t cfs_checksums(cfs_solver* s, double out[4]);
  * No reference counterpart: it calibrates hardware byte counters against a
  * kernel of known traffic. */
 int32_t cfs_calibrate_copy(cfs_solver* s, int32_t repeats);
-/* Number of tiles of the last FP64 stage that were recomputed with IEEE
- * division because they held values outside the range (|v| in [2^-900, 2^1000]
- * or 0) where the kernel's exact division-by-3/6 shortcut applies. */
+/* Number of 15x16-cell tiles of the last FP64 stage holding a value outside
+ * the range (0, or |v| in [2^-900, 2^1000]) in which the kernel's division-free
+ * evaluation of the WENO5 divisions by 3 and 6 is bit-identical to IEEE
+ * division.  Outside it the two differ by at most a few subnormal ulps
+ * (absolute error below 1e-320); 0 means the stage was bit-faithful. */
 int32_t cfs_last_redo_tiles(cfs_solver* s, int32_t* n);
 /* next stage counter (1 + stages done) */
 int32_t cfs_stage_counter(const cfs_solver* s);
-/* Mean duration (ms) of the stage kernel over the last cfs_advance call,
- * measured with HIP events on the handle's stream (0 if none). */
+/* Kernel timing switch (default off): when on, cfs_advance brackets every
+ * stage kernel with HIP events on the handle's stream (a few microseconds of
+ * queue latency per stage) and cfs_last_kernel_ms reports their mean. */
+int32_t cfs_set_kernel_timing(cfs_solver* s, int32_t on);
+/* Mean duration (ms) of the stage kernel over the last cfs_advance call made
+ * with kernel timing on (0 if none). */
 int32_t cfs_last_kernel_ms(const cfs_solver* s, double* ms_per_stage);
 
 #ifdef __cplusplus

@@ -179,26 +179,35 @@ def test_state_sequence_errors():
         s.set_state(cfs.riemann_quadrants(cfg))
 
 
-def test_exact_division_redo_path():
-    # momentum of order 1e-300 in a patch: those tiles fall outside the range of
-    # the kernel's division-by-3/6 shortcut and are recomputed with IEEE division
+def test_constant_division_range_counter():
+    # momentum of order 1e-300 in a patch: outside the range where the kernel's
+    # division-free x/3, x/6 is bit-identical to IEEE division.  The tiles are
+    # counted, and the states stay within the parity tolerance.
     cfg = cfs.make_config(64, 48, 1.0, 0.75, rk=3)
     U = cfs.riemann_quadrants(cfg)
     U[2, 20:30, 10:20] = 1e-300
     U[1, 40:44, 30:34] = -3e-305
     s, o = _pair(cfg, U)
     dt = 0.4 / o.cfl_smax()
-    lib = cfs.library()
-    n = ctypes.c_int32(-1)
-    redo = []
+    counted = []
     for i in range(1, 13):
         s.stage(i, dt)
         o.stage(i, dt)
-        assert lib.cfs_last_redo_tiles(s._h, ctypes.byref(n)) == 0
-        redo.append(n.value)
-        _assert_parity(s.get_state(), o.get_state(), "redo path, stage %d" % i)
-    print("redo tiles per stage:", redo)
-    assert redo[0] > 0
+        counted.append(s.exact_tiles())
+        g, r = s.get_state(), o.get_state()
+        scale = np.maximum(1.0, np.abs(r).reshape(4, -1).max(axis=1))
+        assert np.all(np.abs(g - r).reshape(4, -1).max(axis=1) <= TOL * scale)
+    print("tiles outside the bit-identical range per stage:", counted)
+    assert counted[0] > 0
+    # a normal-range state: no tile counted, bit-identical
+    cfg = cfs.make_config(64, 48, 1.0, 0.75, rk=3)
+    U = cfs.riemann_quadrants(cfg)
+    s, o = _pair(cfg, U)
+    for i in range(1, 7):
+        s.stage(i, dt)
+        o.stage(i, dt)
+        assert s.exact_tiles() == 0
+    _assert_parity(s.get_state(), o.get_state(), "normal range")
 
 
 def test_fp32_bulk_l1_density():
