@@ -1,0 +1,24 @@
+"""Native (C) checks of arithmetic shortcuts used by the kernels.
+
+const_div_check.c: the division-free x/3, x/6 of the FP64 stage kernel
+(q0 = x*z; r = fma(-q0, d, x); q = fma(r, z, q0)) equals IEEE x/d bit for bit on
+random normal-range doubles, integers, quotients next to rounding boundaries
+and signed zeros.
+"""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc not available")
+def test_constant_division_shortcut_is_exact(tmp_path):
+    exe = str(tmp_path / "cdc")
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-o", exe, os.path.join(HERE, "native", "const_div_check.c"),
+                    "-lm"], check=True)
+    r = subprocess.run([exe, "20000000"], stdout=subprocess.PIPE, text=True)
+    assert r.returncode == 0, r.stdout
+    assert r.stdout.strip().splitlines()[-1] == "0"
