@@ -47,7 +47,6 @@ struct StageArgs {
     const double* u1;     // RK snapshot U1 (form 2)
     double* out;          // next state
     const uint8_t* type;  // cell types or nullptr (all fluid)
-    const double* dt;     // device scalar: time step of this stage
     long long plane;      // nx*ny
     int nx, ny;
     int tiles_j;          // tiles along j
@@ -56,11 +55,32 @@ struct StageArgs {
     int hll_bounds;       // 0: reference (momentum, left state in SR), 1: velocity bounds
     int dx_pow2, dy_pow2; // dx (dy) is a power of two: divide by multiplying with 1/dx
     double dx, dy, inv_dx, inv_dy, gamma;
-    int* exact_count;     // tiles of this stage outside the x/3, x/6 shortcut's exact range
-    int* exact_other;     // the previous stage's counter: cleared by this stage
     double c0, c1, c2;
     int bc[4];
     double bcs[4][4];
+    int* exact_count;     // tiles of this stage outside the x/3, x/6 shortcut's exact range
+    int* exact_other;     // the previous stage's counter: cleared by this stage
+    // immersed boundary, filled while staging the tiles next to the body
+    const uint8_t* tile_ib;   // per tile: its load region holds immersed-ghost cells
+    const int* ig_index;      // per cell: row of the IB tables, -1 elsewhere
+    const int* ig_donors;
+    const double* ig_coef;
+    const int* ig_transp;
+    const double* ig_xy;
+    int ib_interp;            // 0: reference LDL^T solve, 1: donor weights
+    // time step
+    int dt_mode;              // 0: *dt given; 1: first stage of a CFL step, dt from acc_in;
+                              // 2: later stage of a CFL step, *dt and *halt from the first
+    double* dt;
+    const unsigned long long* acc_in;   // CFL bound of the state entering the step (bits)
+    unsigned long long* acc_out;        // CFL bound of the state leaving the step
+    int fuse_cfl;             // last stage: accumulate the CFL bound of the new state
+    const double* t_in;       // time entering the step
+    double* t_out;            // time leaving it
+    int* halt;                // 1: this step does nothing (end time reached or bad bound)
+    int* status;              // set to 1 on a non-finite or non-positive CFL bound
+    int* steps;               // steps taken
+    double cfl, t_end;        // t_end <= 0: no end time
 };
 
 // ---------------------------------------------------------------------------
@@ -78,8 +98,7 @@ template <> struct K<double> {
     // ulp(q0)); q0 + r*z then differs from x/d by at most |x/d|*2^-105, while
     // x/d (d = 3*2^k) lies at least 2^(e-53)/3 from every rounding midpoint of
     // its binade 2^e, so the single rounding of the fma gives RN(x/d).  q0 is
-    // returned for zero to keep the sign.  The stage kernel takes this path only
-    // for tiles whose staged values all lie in that range.
+    // returned for zero to keep the sign.
     __device__ static double div3c(double x) {
         const double z = 0x1.5555555555555p-2;
         const double q0 = x * z;
@@ -229,7 +248,8 @@ __device__ __forceinline__ void face_flux(const R* __restrict__ S, const int cel
 // ghost cell (the reference's face walk clamps to it, CellInterface.cpp:172-178):
 // Neumann ghost = copy of the adjacent interior cell (NeummanBoundary.cpp:31-45),
 // Dirichlet ghost = the fixed state (DirichletBoundary.cpp:32-46).
-__device__ __forceinline__ void fetch_cell(const StageArgs& a, int gi, int gj, double v[4]) {
+// Returns the linear index of the interior cell read, -1 for a ghost value.
+__device__ __forceinline__ long long fetch_cell(const StageArgs& a, int gi, int gj, double v[4]) {
     int side = -1;
     if (gi < 0) {
         if (a.bc[0] == BC_PERIODIC) gi += a.nx; else side = 0;
@@ -246,33 +266,198 @@ __device__ __forceinline__ void fetch_cell(const StageArgs& a, int gi, int gj, d
     if (side >= 0 && a.bc[side] == BC_DIRICHLET) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) v[q] = a.bcs[side][q];
-        return;
+        return -1;
     }
     const long long c = (long long)gi * a.ny + gj;
 #pragma unroll
     for (int q = 0; q < 4; ++q) v[q] = a.in[q * a.plane + c];
+    return side >= 0 ? -1 : c;
 }
 
+// ---------------------------------------------------------------------------
+// Immersed-boundary ghost value (Cell::comImmersedGhostPhase, Cell.cpp:172-181
+// -> ImagePoint::InterpolatePhase, ImagePoint.cpp:20-82): 4 donor states; the
+// reference interpolation solves the donor system through its precomputed
+// pivoted LDL^T per variable and evaluates w0*x*y + w1*x + w2*y + w3 at the
+// image point; the positivity-preserving one is sum_k g_k f_k.  The ghost is
+// (rho, -rho*u, -rho*v, E).  FP64 in every precision mode.
+
+__device__ __forceinline__ double pick4(const double x[4], int p) {
+    return p == 0 ? x[0] : p == 1 ? x[1] : p == 2 ? x[2] : x[3];
+}
+__device__ __forceinline__ void put4(double x[4], int p, double v) {
+    x[0] = (p == 0) ? v : x[0];
+    x[1] = (p == 1) ? v : x[1];
+    x[2] = (p == 2) ? v : x[2];
+    x[3] = (p == 3) ? v : x[3];
+}
+
+// Solve with the pivoted LDL^T of the donor matrix, in the order of Eigen's
+// LDLT::solve: transpositions, unit-lower solve (column sweep), D^-1 (zero
+// below DBL_MIN), unit-upper (L^T) solve (row dot products), transpositions
+// in reverse.
+__device__ __forceinline__ void ldlt_solve(const double* l, const int* tr, double x[4]) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int p = tr[k];
+        const double xk = x[k], xp = pick4(x, p);
+        put4(x, p, xk);
+        x[k] = xp;
+    }
+    x[1] -= x[0] * l[0];
+    x[2] -= x[0] * l[1];
+    x[3] -= x[0] * l[3];
+    x[2] -= x[1] * l[2];
+    x[3] -= x[1] * l[4];
+    x[3] -= x[2] * l[5];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const double d = l[6 + i];
+        x[i] = (__builtin_fabs(d) > 2.2250738585072014e-308) ? x[i] / d : 0.0;
+    }
+    x[2] -= l[5] * x[3];
+    x[1] -= l[2] * x[2] + l[4] * x[3];
+    x[0] -= (l[0] * x[1] + l[1] * x[2]) + l[3] * x[3];
+#pragma unroll
+    for (int k = 3; k >= 0; --k) {
+        const int p = tr[k];
+        const double xk = x[k], xp = pick4(x, p);
+        put4(x, p, xk);
+        x[k] = xp;
+    }
+}
+
+__device__ __forceinline__ void ghost_value(const double* __restrict__ U, long long plane, int interp,
+                                            const int* dn, const double* l, const int* tr, const double* xy,
+                                            double v[4]) {
+    // one variable at a time keeps the register footprint small
+    const long long c0 = dn[0], c1 = dn[1], c2 = dn[2], c3 = dn[3];
+    double val[4];
+#pragma unroll 1
+    for (int q = 0; q < 4; ++q) {
+        const double* Uq = U + q * plane;
+        double w[4] = {Uq[c0], Uq[c1], Uq[c2], Uq[c3]};
+        if (interp == 0) {
+            ldlt_solve(l, tr, w);
+            const double x = xy[0], y = xy[1];
+            val[q] = ((w[0] * x * y + w[1] * x) + w[2] * y) + w[3];
+        } else {
+            val[q] = ((l[0] * w[0] + l[1] * w[1]) + l[2] * w[2]) + l[3] * w[3];
+        }
+    }
+    v[0] = val[0];
+    v[1] = -val[1];
+    v[2] = -val[2];
+    v[3] = val[3];
+}
+
+// CFL bound of one cell: (a_x + c)/dx + (a_y + c)/dy, c = sqrt(gamma p/rho),
+// a_x = max(|u|, |rho u|) with the reference HLL bounds (they use the
+// momentum), |u| with velocity bounds.
+__device__ __forceinline__ double cell_speed(double rho, double mu, double mv, double E, double gamma, double dx,
+                                             double dy, int hll_bounds) {
+    const double p = (gamma - 1) * (E - 0.5 * (mu * mu + mv * mv) / rho);
+    const double cs = __builtin_sqrt(gamma * p / rho);
+    const double u = mu / rho, v = mv / rho;
+    double ax = __builtin_fabs(u), ay = __builtin_fabs(v);
+    if (hll_bounds == 0) {
+        if (ax < __builtin_fabs(mu)) ax = __builtin_fabs(mu);
+        if (ay < __builtin_fabs(mv)) ay = __builtin_fabs(mv);
+    }
+    return (ax + cs) / dx + (ay + cs) / dy;
+}
+
+__device__ __forceinline__ double max_nan(double a, double b) { return (b > a || b != b) ? b : a; }
+
+// Block-wide max (NaN wins) -> atomicMax on the bit pattern.  The atomic is
+// skipped when the accumulator already holds at least this block's value: the
+// accumulator only grows, so a stale read can only cost an extra atomic, never
+// lose a maximum -- and thousands of workgroups do not queue on one address.
+__device__ __forceinline__ void block_max_to(double m, unsigned long long* acc, double* wmax) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) m = max_nan(m, __shfl_xor(m, off, 64));
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) wmax[w] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = wmax[0];
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) b = max_nan(b, wmax[k]);
+        const unsigned long long mine = (unsigned long long)__double_as_longlong(b);
+        const unsigned long long seen = __hip_atomic_load(acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (mine > seen) atomicMax(acc, mine);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // One tile: stage it (with halo) in LDS, compute its 511 faces, update its
 // cells.  Tiles holding a value outside the range where the WENO5 constant
 // divisions are bit-identical to IEEE division are counted in exact_count.
 template <typename R>
-__device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R* F) {
+__device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R* F, double* wmax) {
     const int tid = threadIdx.x;
     const int ti = tile / a.tiles_j;
     const int tj = tile - ti * a.tiles_j;
     const int i0 = ti * TI, j0 = tj * TJ;
-    const double dt = *a.dt;
-    int unsafe = 0;
+
     if (tile == 0 && tid == 0) *a.exact_other = 0;   // the previous stage's counter, consumed
 
+    // ---- time step ----
+    double dt;
+    if (a.dt_mode == 1) {
+        // first stage of a CFL step: every workgroup derives the same dt from
+        // the bound accumulated by the previous step's last stage
+        const double t = *a.t_in;
+        bool stop = a.t_end > 0.0 && !(t < a.t_end);
+        bool bad = false;
+        double d = 0.0;
+        if (!stop) {
+            const double s = __longlong_as_double((long long)*a.acc_in);
+            d = a.cfl / s;
+            if (!(s > 0.0) || !(d < 1.0e300)) {
+                bad = true;
+                stop = true;
+                d = 0.0;
+            } else if (a.t_end > 0.0) {
+                const double rem = a.t_end - t;
+                if (!(d < rem)) d = rem > 0.0 ? rem : 0.0;
+            }
+        }
+        if (tile == 0 && tid == 0) {
+            *a.dt = d;
+            *a.t_out = t + d;
+            *a.halt = stop ? 1 : 0;
+            if (bad) *a.status = 1;
+            if (!stop) *a.steps += 1;
+            *a.acc_out = 0ull;   // consumed two steps ago; this step's last stage fills it
+        }
+        if (stop) return;
+        dt = d;
+    } else if (a.dt_mode == 2) {
+        if (*a.halt) return;
+        dt = *a.dt;
+    } else {
+        dt = *a.dt;
+    }
+    int unsafe = 0;
+
     // ---- tile + halo -> LDS (corners are never read) ----
+    const bool near_ib = a.tile_ib != nullptr && a.tile_ib[tile];
     for (int s = tid; s < SR * SC; s += BLOCK) {
         const int r = s / SC, c = s - r * SC;
         const bool xs = (c >= HALO && c < HALO + TJ), ys = (r >= HALO && r < HALO + TI);
         if (!(xs || ys)) continue;
         double v[4];
-        fetch_cell(a, i0 - HALO + r, j0 - HALO + c, v);
+        const long long cell = fetch_cell(a, i0 - HALO + r, j0 - HALO + c, v);
+        if (near_ib && cell >= 0 && a.type[cell] == 2) {
+            // immersed ghost: its value at the start of this stage, from its donors
+            const int k = a.ig_index[cell];
+            ghost_value(a.in, a.plane, a.ib_interp, a.ig_donors + 4 * k, a.ig_coef + 10 * k, a.ig_transp + 4 * k,
+                        a.ig_xy + 2 * k, v);
+            if (xs && ys && i0 - HALO + r < a.nx && j0 - HALO + c < a.ny) {   // one of the tile's own cells
+#pragma unroll
+                for (int q = 0; q < 4; ++q) a.out[q * a.plane + cell] = v[q];
+            }
+        }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             S[s * 4 + q] = (R)v[q];
@@ -325,55 +510,63 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
 
     // ---- cell update: flux divergence (Cell::comRHS, Cell.cpp:110-125) and the
     // TVD-RK combination (TVD2rdRK.cpp:3-31, TVD3rdRK.cpp:3-48) ----
+    double smax = 0.0;
     if (tid < TI * TJ) {
         const int il = tid / TJ, jl = tid - il * TJ;
         const int i = i0 + il, j = j0 + jl;
         if (i < a.nx && j < a.ny) {
             const long long c = (long long)i * a.ny + j;
-            const bool fluid = (a.type == nullptr) || (a.type[c] == 0);
+            const int ty = (a.type == nullptr) ? 0 : a.type[c];
             const R rdx = (R)a.dx, rdy = (R)a.dy;
             const R idx = (R)a.inv_dx, idy = (R)a.inv_dy;
             const double c2dt = a.c2 * dt;
+            double res[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 // the FP64 tile holds the exact state; FP32-bulk re-reads it
                 const double u = (sizeof(R) == 8) ? (double)S[((il + HALO) * SC + jl + HALO) * 4 + q]
                                                   : a.in[q * a.plane + c];
-                double res = u;
-                if (fluid) {
+                res[q] = u;
+                if (ty == 0) {
                     const R fl = F[(il * TJ + jl) * 4 + q];
                     const R fr = F[((il + 1) * TJ + jl) * 4 + q];
                     const R gb = F[(NXF + il * (TJ + 1) + jl) * 4 + q];
                     const R gt = F[(NXF + il * (TJ + 1) + jl + 1) * 4 + q];
                     // x/d == x*(1/d) bit for bit when d is a power of two
                     const R tx = a.dx_pow2 ? (-(fr - fl)) * idx : K<R>::div(-(fr - fl), rdx);
-                    const R ty = a.dy_pow2 ? (gt - gb) * idy : K<R>::div(gt - gb, rdy);
-                    const R rhs = tx - ty;
+                    const R tyy = a.dy_pow2 ? (gt - gb) * idy : K<R>::div(gt - gb, rdy);
+                    const R rhs = tx - tyy;
                     const double L = (double)rhs;
-                    if (a.form == 0) res = a.c1 * u + c2dt * L;
-                    else if (a.form == 1) res = (a.c0 * a.u0[q * a.plane + c] + a.c1 * u) + c2dt * L;
-                    else res = (a.c0 * a.u0[q * a.plane + c] + a.c1 * a.u1[q * a.plane + c]) + c2dt * L;
+                    if (a.form == 0) res[q] = a.c1 * u + c2dt * L;
+                    else if (a.form == 1) res[q] = (a.c0 * a.u0[q * a.plane + c] + a.c1 * u) + c2dt * L;
+                    else res[q] = (a.c0 * a.u0[q * a.plane + c] + a.c1 * a.u1[q * a.plane + c]) + c2dt * L;
                 }
-                a.out[q * a.plane + c] = res;
             }
+            // immersed ghosts were written while staging the tile
+            if (ty != 2) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) a.out[q * a.plane + c] = res[q];
+            }
+            if (a.fuse_cfl && ty == 0)
+                smax = cell_speed(res[0], res[1], res[2], res[3], a.gamma, a.dx, a.dy, a.hll_bounds);
         }
     }
+    if (a.fuse_cfl) block_max_to(smax, a.acc_out, wmax);
 }
 
-// The stage kernel: one workgroup per tile.
+// The stage kernel: one workgroup per tile.  FP64: at most 96 VGPRs so that
+// five workgroups (20 waves) share a CU; FP32: six (the most without spills).
 template <typename R>
-__global__ __launch_bounds__(BLOCK) void stage_kernel(StageArgs a) {
+__global__ __launch_bounds__(BLOCK, sizeof(R) == 8 ? 5 : 6) void stage_kernel(StageArgs a) {
     __shared__ R S[SR * SC * 4];
     __shared__ R F[(NXF + NYF) * 4];
-    stage_tile<R>(a, blockIdx.x, S, F);
+    __shared__ double wmax[BLOCK / 64];
+    stage_tile<R>(a, blockIdx.x, S, F, wmax);
 }
 
 // ---------------------------------------------------------------------------
-// Immersed-boundary fill (Cell::comImmersedGhostPhase, Cell.cpp:172-181 ->
-// ImagePoint::InterpolatePhase, ImagePoint.cpp:20-82): each immersed-ghost
-// cell solves the donor system through its precomputed pivoted LDL^T, per
-// variable, evaluates w0*x*y + w1*x + w2*y + w3 at its image point and
-// stores (rho, -rho*u, -rho*v, E).  FP64 in every precision mode.
+// Immersed-boundary fill of a whole state (the reference's prologue fill,
+// Run.cpp:106-112); during stages the fill happens in the tile load.
 struct IBArgs {
     double* U;
     long long plane;
@@ -382,148 +575,37 @@ struct IBArgs {
     const int* cells;
     const int* donors;
     const double* ipxy;
-    const double* ldl;    // L10 L20 L21 L30 L31 L32 D0 D1 D2 D3
+    const double* ldl;    // L10 L20 L21 L30 L31 L32 D0 D1 D2 D3, or 4 weights
     const int* transp;
 };
-
-__device__ __forceinline__ double pick4(const double x[4], int p) {
-    return p == 0 ? x[0] : p == 1 ? x[1] : p == 2 ? x[2] : x[3];
-}
-__device__ __forceinline__ void put4(double x[4], int p, double v) {
-    x[0] = (p == 0) ? v : x[0];
-    x[1] = (p == 1) ? v : x[1];
-    x[2] = (p == 2) ? v : x[2];
-    x[3] = (p == 3) ? v : x[3];
-}
-
-// Solve with the pivoted LDL^T of the donor matrix, in the order of Eigen's
-// LDLT::solve: transpositions, unit-lower solve (column sweep), D^-1 (zero
-// below DBL_MIN), unit-upper (L^T) solve (row dot products), transpositions
-// in reverse.
-__device__ __forceinline__ void ldlt_solve(const double* l, const int* tr, double x[4]) {
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int p = tr[k];
-        const double xk = x[k], xp = pick4(x, p);
-        put4(x, p, xk);
-        x[k] = xp;
-    }
-    x[1] -= x[0] * l[0];
-    x[2] -= x[0] * l[1];
-    x[3] -= x[0] * l[3];
-    x[2] -= x[1] * l[2];
-    x[3] -= x[1] * l[4];
-    x[3] -= x[2] * l[5];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const double d = l[6 + i];
-        x[i] = (__builtin_fabs(d) > 2.2250738585072014e-308) ? x[i] / d : 0.0;
-    }
-    x[2] -= l[5] * x[3];
-    x[1] -= l[2] * x[2] + l[4] * x[3];
-    x[0] -= (l[0] * x[1] + l[1] * x[2]) + l[3] * x[3];
-#pragma unroll
-    for (int k = 3; k >= 0; --k) {
-        const int p = tr[k];
-        const double xk = x[k], xp = pick4(x, p);
-        put4(x, p, xk);
-        x[k] = xp;
-    }
-}
 
 __global__ __launch_bounds__(256) void ib_fill_kernel(IBArgs a) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= a.n) return;
-    const int* dn = a.donors + 4 * k;
-    double f[4][4];
-#pragma unroll
-    for (int d = 0; d < 4; ++d) {
-        const long long c = dn[d];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) f[q][d] = a.U[q * a.plane + c];
-    }
-    const double* l = a.ldl + 10 * k;
-    double val[4];
-    if (a.interp == 0) {
-        const int* tr = a.transp + 4 * k;
-        const double x = a.ipxy[2 * k], y = a.ipxy[2 * k + 1];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            double w[4] = {f[q][0], f[q][1], f[q][2], f[q][3]};
-            ldlt_solve(l, tr, w);
-            val[q] = ((w[0] * x * y + w[1] * x) + w[2] * y) + w[3];
-        }
-    } else {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) val[q] = ((l[0] * f[q][0] + l[1] * f[q][1]) + l[2] * f[q][2]) + l[3] * f[q][3];
-    }
+    double v[4];
+    ghost_value(a.U, a.plane, a.interp, a.donors + 4 * k, a.ldl + 10 * k, a.transp + 4 * k, a.ipxy + 2 * k, v);
     const long long c = a.cells[k];
-    a.U[c] = val[0];
-    a.U[a.plane + c] = -val[1];
-    a.U[2 * a.plane + c] = -val[2];
-    a.U[3 * a.plane + c] = val[3];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) a.U[q * a.plane + c] = v[q];
 }
 
 // ---------------------------------------------------------------------------
-// CFL reduction: s = max over fluid cells of (|u|+c)/dx + (|v|+c)/dy.  Every
-// wavefront reduces with shuffles, every workgroup through LDS, and one
-// atomicMax on the bit pattern (non-negative doubles order as unsigned
-// integers; a NaN wins) combines the workgroups -- exact, so order-free.
-__device__ __forceinline__ double max_nan(double a, double b) { return (b > a || b != b) ? b : a; }
-
+// CFL reduction over a whole state (first step of an advance, cfs_compute_dt):
+// wave64 shuffles, LDS, one atomicMax on the bit pattern (non-negative doubles
+// order as unsigned integers; a NaN wins) -- exact, so order-free.
 __global__ __launch_bounds__(256) void cfl_kernel(const double* __restrict__ U, long long plane,
                                                   const uint8_t* __restrict__ type, double gamma,
                                                   double dx, double dy, int hll_bounds,
                                                   unsigned long long* acc) {
+    __shared__ double wmax[4];
     double m = 0.0;
     for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < plane;
          c += (long long)gridDim.x * blockDim.x) {
         if (type && type[c] != 0) continue;
-        const double rho = U[c], mu = U[plane + c], mv = U[2 * plane + c], E = U[3 * plane + c];
-        const double p = (gamma - 1) * (E - 0.5 * (mu * mu + mv * mv) / rho);
-        const double cs = __builtin_sqrt(gamma * p / rho);
-        const double u = mu / rho, v = mv / rho;
-        double ax = __builtin_fabs(u), ay = __builtin_fabs(v);
-        if (hll_bounds == 0) {   // the reference HLL's bounds use the momentum
-            if (ax < __builtin_fabs(mu)) ax = __builtin_fabs(mu);
-            if (ay < __builtin_fabs(mv)) ay = __builtin_fabs(mv);
-        }
-        const double s = (ax + cs) / dx + (ay + cs) / dy;
-        m = max_nan(m, s);
+        m = max_nan(m, cell_speed(U[c], U[plane + c], U[2 * plane + c], U[3 * plane + c], gamma, dx, dy,
+                                  hll_bounds));
     }
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) m = max_nan(m, __shfl_xor(m, off, 64));
-    __shared__ double wmax[4];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) wmax[w] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double b = wmax[0];
-        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) b = max_nan(b, wmax[k]);
-        atomicMax(acc, (unsigned long long)__double_as_longlong(b));
-    }
-}
-
-// dt = cfl / s, optionally clipped to t_end - t; acc is cleared for the next
-// reduction (stream order makes this safe without a memset).
-struct DtArgs {
-    unsigned long long* acc;
-    double* dt;
-    double* t;          // simulated time
-    double cfl, t_end;  // t_end <= 0: no clipping
-    int* status;        // set to 1 when s is not finite or not positive
-};
-__global__ void dt_finalize_kernel(DtArgs a) {
-    const double s = __longlong_as_double((long long)*a.acc);
-    *a.acc = 0ull;
-    double dt = a.cfl / s;
-    if (!(s > 0.0) || !(dt < 1.0e300)) { *a.status = 1; dt = 0.0; }
-    if (a.t_end > 0.0) {
-        const double rem = a.t_end - *a.t;
-        if (!(dt < rem)) dt = rem > 0.0 ? rem : 0.0;
-    }
-    *a.dt = dt;
-    *a.t = *a.t + dt;
+    block_max_to(m, acc, wmax);
 }
 
 // plain streaming copy (16-byte vectors), the byte-counter calibration kernel

@@ -166,6 +166,43 @@ def test_cfl_dt_and_advance():
     assert abs(t2 - (t + 0.01)) <= 1e-15 and done2 < 1000
 
 
+def test_cylinder_cfl_advance_fused_ib_and_bound():
+    # CFL steps with the immersed-boundary fill done in the tile load and the
+    # bound of each new state accumulated in the last stage; graph replay
+    cfg = cfs.config_cylinder(128, 64, 4.0, 2.0, freestream=SUBSONIC, positivity=0, hll_bounds=0)
+    ib = cfs.circle_setup(cfg, 1.0, 1.0, 0.25, interp=cfs.IB_BILINEAR_POSITIVE)
+    U = cfs.cylinder_freestream(cfg, ib["cell_type"], freestream=SUBSONIC)
+    s, o = _pair(cfg, U, ib)
+    t, done = s.advance(30, dt=0.0, cfl=0.5)
+    tt = 0.0
+    for n in range(30):
+        dt = 0.5 / o.cfl_smax()
+        o.step(dt)
+        tt += dt
+    assert done == 30 and t == tt
+    _assert_parity(s.get_state(), o.get_state(), "cylinder CFL advance")
+
+
+def test_graph_replay_equals_direct_launches():
+    cfg = cfs.config_riemann(48, rk=3)
+    U = cfs.riemann_quadrants(cfg)
+    a = cfs.Solver(cfg)
+    b = cfs.Solver(cfg)
+    a.set_state(U)
+    b.set_state(U)
+    b.set_kernel_timing(True)      # event-bracketed direct launches, no graph
+    ta, na = a.advance(11, dt=0.0, cfl=0.5)
+    tb, nb = b.advance(11, dt=0.0, cfl=0.5)
+    assert (ta, na) == (tb, nb)
+    assert b.last_kernel_ms() > 0
+    assert np.array_equal(a.get_state(), b.get_state())
+    # and again from the new state (cached graph, other parity)
+    ta, na = a.advance(6, dt=0.0, cfl=0.5, t=ta)
+    tb, nb = b.advance(6, dt=0.0, cfl=0.5, t=tb)
+    assert (ta, na) == (tb, nb)
+    assert np.array_equal(a.get_state(), b.get_state())
+
+
 def test_state_sequence_errors():
     cfg = cfs.config_riemann(16)
     s = cfs.Solver(cfg)
