@@ -162,6 +162,48 @@ def cylinder_freestream(cfg, cell_type, freestream=CYLINDER_FREESTREAM):
     return U
 
 
+def splitmix64(z):
+    """SplitMix64 finaliser (Steele, Lea, Flood 2014) on uint64 arrays."""
+    with np.errstate(over="ignore"):   # arithmetic mod 2^64
+        z = np.asarray(z, dtype=np.uint64) + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def cell_noise(seed, nx, ny):
+    """Counter-based uniform noise in [-1, 1) per cell: a function of
+    (seed, i, j) only, so any producer of the same grid draws the same values.
+    u = top 53 bits of splitmix64(splitmix64(seed) ^ (i << 32 | j)) / 2^53."""
+    i = np.arange(nx, dtype=np.uint64)[:, None]
+    j = np.arange(ny, dtype=np.uint64)[None, :]
+    key = splitmix64(np.uint64(seed)) ^ ((i << np.uint64(32)) | j)
+    u = (splitmix64(key) >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+    return 2.0 * u - 1.0
+
+
+KH_SEED = 0x5EED
+
+
+def config_kelvin_helmholtz(n=4096, precision=PREC_FP64, rk=3, mu=1e-5, prandtl=0.72, ma_ref=1.0,
+                            gamma=1.4):
+    """Viscous shear layer on the periodic unit square: Navier-Stokes terms
+    with constant mu = 1/Re (Re = 1e5), Pr = 0.72, T = gamma Ma^2 p/rho."""
+    return make_config(n, n, 1.0, 1.0, bc=(BC_PERIODIC,) * 4, gamma=gamma, rk=rk, precision=precision,
+                       viscous=1, mu=mu, prandtl=prandtl, ma_ref=ma_ref)
+
+
+def kelvin_helmholtz(cfg, seed=KH_SEED, amp=0.01, noise=1e-3):
+    """rho = 2, u = +0.5 in 0.25 < y < 0.75; rho = 1, u = -0.5 elsewhere;
+    p = 2.5; v = amp*sin(4 pi x) + noise * cell_noise(seed).  [4][nx][ny]."""
+    x, y = cfg.centres()
+    band = (y > 0.25) & (y < 0.75)
+    rho = np.where(band, 2.0, 1.0)
+    u = np.where(band, 0.5, -0.5)
+    v = amp * np.sin(4.0 * np.pi * x) + noise * cell_noise(seed, cfg.nx, cfg.ny)
+    return np.stack(conserved(rho, u, v, np.full_like(rho, 2.5), gamma=cfg.gamma))
+
+
 # ---------------------------------------------------------------------------
 _LIB = None
 

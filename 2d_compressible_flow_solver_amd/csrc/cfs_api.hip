@@ -104,7 +104,11 @@ int check_config(const cfs_config* c) {
     if (c->precision != CFS_PREC_FP64 && c->precision != CFS_PREC_FP32_BULK) return CFS_E_ARG;
     if (c->positivity != 0 && c->positivity != 1) return CFS_E_ARG;
     if (c->hll_bounds != 0 && c->hll_bounds != 1) return CFS_E_ARG;
-    if (c->viscous != 0) return CFS_E_UNSUPPORTED;
+    if (c->viscous != 0 && c->viscous != 1) return CFS_E_ARG;
+    if (c->viscous) {
+        if (!(c->mu >= 0) || !(c->mu < 1e300) || !(c->prandtl > 0) || !(c->ma_ref > 0)) return CFS_E_ARG;
+        if (!(c->prandtl < 1e300) || !(c->ma_ref < 1e150)) return CFS_E_ARG;
+    }
     return CFS_OK;
 }
 
@@ -166,6 +170,11 @@ int launch_stage(cfs_solver* s, int counter, bool timed, const StageTime& tm) {
     a.gamma = c.gamma;
     a.positivity = c.positivity;
     a.hll_bounds = c.hll_bounds;
+    if (c.viscous) {
+        a.mu = c.mu;
+        a.ma2 = c.ma_ref * c.ma_ref;
+        a.kappa = c.mu / ((c.gamma - 1) * (c.ma_ref * c.ma_ref) * c.prandtl);
+    }
     a.exact_count = s->exact + s->parity;
     a.exact_other = s->exact + (1 - s->parity);
     for (int k = 0; k < 4; ++k) {
@@ -211,10 +220,13 @@ int launch_stage(cfs_solver* s, int counter, bool timed, const StageTime& tm) {
         s->ev.push_back(e1);
         HIP_OK(hipEventRecord(e0, s->stream));
     }
-    if (c.precision == CFS_PREC_FP64)
-        hipLaunchKernelGGL(stage_kernel<double>, grid, dim3(BLOCK), 0, s->stream, a);
-    else
-        hipLaunchKernelGGL(stage_kernel<float>, grid, dim3(BLOCK), 0, s->stream, a);
+    if (c.precision == CFS_PREC_FP64) {
+        if (c.viscous) hipLaunchKernelGGL((stage_kernel<double, true>), grid, dim3(BLOCK), 0, s->stream, a);
+        else hipLaunchKernelGGL((stage_kernel<double, false>), grid, dim3(BLOCK), 0, s->stream, a);
+    } else {
+        if (c.viscous) hipLaunchKernelGGL((stage_kernel<float, true>), grid, dim3(BLOCK), 0, s->stream, a);
+        else hipLaunchKernelGGL((stage_kernel<float, false>), grid, dim3(BLOCK), 0, s->stream, a);
+    }
     HIP_OK(hipGetLastError());
     if (timed) HIP_OK(hipEventRecord(s->ev.back(), s->stream));
     s->parity ^= 1;

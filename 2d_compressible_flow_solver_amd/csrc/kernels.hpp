@@ -81,6 +81,15 @@ struct StageArgs {
     int* status;              // set to 1 on a non-finite or non-positive CFL bound
     int* steps;               // steps taken
     double cfl, t_end;        // t_end <= 0: no end time
+    // viscous terms
+    double mu, kappa, ma2;    // viscosity, mu/((gamma-1) Ma^2 Pr), Ma^2
+};
+
+// LDS of the viscous variant: primitive u, v, T of every staged cell and the
+// cell gradients d/dy (x-face cross terms) and d/dx (y-face cross terms) of u, v
+template <typename R> struct ViscLDS {
+    R pu[SR * SC], pv[SR * SC], pT[SR * SC];
+    R gyu[SR * SC], gyv[SR * SC], gxu[SR * SC], gxv[SR * SC];
 };
 
 // ---------------------------------------------------------------------------
@@ -392,8 +401,82 @@ __device__ __forceinline__ void block_max_to(double m, unsigned long long* acc, 
 // One tile: stage it (with halo) in LDS, compute its 511 faces, update its
 // cells.  Tiles holding a value outside the range where the WENO5 constant
 // divisions are bit-identical to IEEE division are counted in exact_count.
+// ---------------------------------------------------------------------------
+// Viscous terms: textbook compressible Navier-Stokes (constant mu, Prandtl
+// number, T = gamma Ma^2 p/rho as EulerPhase.cpp:137) on the stencil tables
+// of source/gradient/FourthOrderCS.h:27-42 (whose own viscous code is never
+// called, Run.cpp:177,188): 4th-order central cell gradients (theta_c),
+// 6th-order staggered derivatives at i-3/2, i-1/2, i+1/2 (theta_r), 4-point
+// interpolation of cell gradients and velocities to those points (zeta_r, the
+// i-3/2 row corrected to its Lagrange weights [5,15,-5,1]/16), combined with
+// (-1, 26, -1)/24 (FourthOrderCS.cpp:77-112).  Face-normal frame: n, t.
+
+template <typename R> struct VT;
+template <> struct VT<double> {
+    static constexpr double thr[3][6] = {
+        {71.0 / 1920, -141.0 / 128, 69.0 / 64, 1.0 / 192, -3.0 / 128, 3.0 / 640},
+        {-3.0 / 640, 25.0 / 384, -75.0 / 64, 75.0 / 64, -25.0 / 384, 3.0 / 640},
+        {-3.0 / 640, 3.0 / 128, -1.0 / 192, -69.0 / 64, 141.0 / 128, -71.0 / 1920}};
+    static constexpr double thc[5] = {1.0 / 12, -8.0 / 12, 0.0, 8.0 / 12, -1.0 / 12};
+    static constexpr double zr[3][4] = {
+        {5.0 / 16, 15.0 / 16, -5.0 / 16, 1.0 / 16}, {-1.0 / 16, 9.0 / 16, 9.0 / 16, -1.0 / 16},
+        {1.0 / 16, -5.0 / 16, 15.0 / 16, 5.0 / 16}};
+};
+template <> struct VT<float> {
+    static constexpr float thr[3][6] = {
+        {71.0f / 1920, -141.0f / 128, 69.0f / 64, 1.0f / 192, -3.0f / 128, 3.0f / 640},
+        {-3.0f / 640, 25.0f / 384, -75.0f / 64, 75.0f / 64, -25.0f / 384, 3.0f / 640},
+        {-3.0f / 640, 3.0f / 128, -1.0f / 192, -69.0f / 64, 141.0f / 128, -71.0f / 1920}};
+    static constexpr float thc[5] = {1.0f / 12, -8.0f / 12, 0.0f, 8.0f / 12, -1.0f / 12};
+    static constexpr float zr[3][4] = {
+        {5.0f / 16, 15.0f / 16, -5.0f / 16, 1.0f / 16}, {-1.0f / 16, 9.0f / 16, 9.0f / 16, -1.0f / 16},
+        {1.0f / 16, -5.0f / 16, 15.0f / 16, 5.0f / 16}};
+};
+
+// net viscous flux (0, normal-momentum, tangential-momentum, energy) of a face
+// whose 6 stencil cells have LDS cell indices cs[k]
 template <typename R>
-__device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R* F, double* wmax) {
+__device__ __forceinline__ void viscous_flux(const ViscLDS<R>& V, const int cs[6], bool xface, R h, R mu, R kappa,
+                                             R out[3]) {
+    const R* un = xface ? V.pu : V.pv;
+    const R* ut = xface ? V.pv : V.pu;
+    const R* gn = xface ? V.gyu : V.gxv;   // tangential derivative of the normal velocity
+    const R* gt = xface ? V.gyv : V.gxu;   // tangential derivative of the tangential velocity
+    R acc[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        R dn_un = R(0), dn_ut = R(0), dn_T = R(0);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            dn_un += VT<R>::thr[r][k] * un[cs[k]];
+            dn_ut += VT<R>::thr[r][k] * ut[cs[k]];
+            dn_T += VT<R>::thr[r][k] * V.pT[cs[k]];
+        }
+        dn_un = K<R>::div(dn_un, h);
+        dn_ut = K<R>::div(dn_ut, h);
+        dn_T = K<R>::div(dn_T, h);
+        R dt_un = R(0), dt_ut = R(0), unf = R(0), utf = R(0);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            dt_un += VT<R>::zr[r][m] * gn[cs[1 + m]];
+            dt_ut += VT<R>::zr[r][m] * gt[cs[1 + m]];
+            unf += VT<R>::zr[r][m] * un[cs[1 + m]];
+            utf += VT<R>::zr[r][m] * ut[cs[1 + m]];
+        }
+        const R tnn = mu * ((R(4.0) / R(3.0)) * dn_un - (R(2.0) / R(3.0)) * dt_ut);
+        const R tnt = mu * (dt_un + dn_ut);
+        acc[r][0] = tnn;
+        acc[r][1] = tnt;
+        acc[r][2] = (unf * tnn + utf * tnt) + kappa * dn_T;
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+        out[q] = ((R(-1.0) / R(24.0)) * acc[0][q] + (R(26.0) / R(24.0)) * acc[1][q]) + (R(-1.0) / R(24.0)) * acc[2][q];
+}
+
+template <typename R, bool VISC>
+__device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R* F, double* wmax,
+                                           ViscLDS<R>* V) {
     const int tid = threadIdx.x;
     const int ti = tile / a.tiles_j;
     const int tj = tile - ti * a.tiles_j;
@@ -445,7 +528,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
     for (int s = tid; s < SR * SC; s += BLOCK) {
         const int r = s / SC, c = s - r * SC;
         const bool xs = (c >= HALO && c < HALO + TJ), ys = (r >= HALO && r < HALO + TI);
-        if (!(xs || ys)) continue;
+        if (!VISC && !(xs || ys)) continue;   // the viscous cross terms read the corners
         double v[4];
         const long long cell = fetch_cell(a, i0 - HALO + r, j0 - HALO + c, v);
         if (near_ib && cell >= 0 && a.type[cell] == 2) {
@@ -467,6 +550,45 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
     // one barrier + OR over the tile: count tiles outside the range where the
     // constant-division shortcut is bit-identical to IEEE division
     if (__syncthreads_or(unsafe) && tid == 0) atomicAdd(a.exact_count, 1);
+
+    if constexpr (VISC) {
+        // primitive variables of every staged cell (EulerPhase.cpp:130-137 forms)
+        const R g = (R)a.gamma, gm1 = g - R(1), gma2 = g * (R)a.ma2;
+        for (int s = tid; s < SR * SC; s += BLOCK) {
+            const R rho = S[s * 4], mx = S[s * 4 + 1], my = S[s * 4 + 2], E = S[s * 4 + 3];
+            const R p = gm1 * (E - K<R>::div(K<R>::half * (mx * mx + my * my), rho));
+            V->pu[s] = K<R>::div(mx, rho);
+            V->pv[s] = K<R>::div(my, rho);
+            V->pT[s] = K<R>::div(gma2 * p, rho);
+        }
+        __syncthreads();
+        // 4th-order central cell gradients wherever the 5-cell stencil is staged
+        const R rdx = (R)a.dx, rdy = (R)a.dy;
+        for (int s = tid; s < SR * SC; s += BLOCK) {
+            const int r = s / SC, c = s - r * SC;
+            if (c >= 2 && c < SC - 2) {
+                R du = R(0), dv = R(0);
+#pragma unroll
+                for (int k = 0; k < 5; ++k) {
+                    du += VT<R>::thc[k] * V->pu[s - 2 + k];
+                    dv += VT<R>::thc[k] * V->pv[s - 2 + k];
+                }
+                V->gyu[s] = K<R>::div(du, rdy);
+                V->gyv[s] = K<R>::div(dv, rdy);
+            }
+            if (r >= 2 && r < SR - 2) {
+                R du = R(0), dv = R(0);
+#pragma unroll
+                for (int k = 0; k < 5; ++k) {
+                    du += VT<R>::thc[k] * V->pu[s + (k - 2) * SC];
+                    dv += VT<R>::thc[k] * V->pv[s + (k - 2) * SC];
+                }
+                V->gxu[s] = K<R>::div(du, rdx);
+                V->gxv[s] = K<R>::div(dv, rdx);
+            }
+        }
+        __syncthreads();
+    }
 
     // ---- faces ----
     const bool px = a.bc[0] == BC_PERIODIC, py = a.bc[2] == BC_PERIODIC;
@@ -501,6 +623,17 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
         }
         R fl[4];
         face_flux<R>(S, cell, nrm, tng, g, a.positivity, a.hll_bounds, fl);
+        if constexpr (VISC) {   // net flux = inviscid - viscous (mass: minus zero)
+            int cs[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) cs[k] = cell[k] >> 2;
+            const bool xface = s < NXF;
+            R fv[3];
+            viscous_flux<R>(*V, cs, xface, xface ? (R)a.dx : (R)a.dy, (R)a.mu, (R)a.kappa, fv);
+            fl[1] = fl[1] - fv[0];
+            fl[2] = fl[2] - fv[1];
+            fl[3] = fl[3] - fv[2];
+        }
         F[s * 4 + 0] = fl[0];
         F[s * 4 + nrm] = fl[1];
         F[s * 4 + tng] = fl[2];
@@ -556,12 +689,18 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
 
 // The stage kernel: one workgroup per tile.  FP64: at most 96 VGPRs so that
 // five workgroups (20 waves) share a CU; FP32: six (the most without spills).
-template <typename R>
-__global__ __launch_bounds__(BLOCK, sizeof(R) == 8 ? 5 : 6) void stage_kernel(StageArgs a) {
+// The viscous variant's LDS (57 KB in FP64) allows two workgroups per CU.
+template <typename R, bool VISC>
+__global__ __launch_bounds__(BLOCK, VISC ? 2 : (sizeof(R) == 8 ? 5 : 6)) void stage_kernel(StageArgs a) {
     __shared__ R S[SR * SC * 4];
     __shared__ R F[(NXF + NYF) * 4];
     __shared__ double wmax[BLOCK / 64];
-    stage_tile<R>(a, blockIdx.x, S, F, wmax);
+    if constexpr (VISC) {
+        __shared__ ViscLDS<R> V;
+        stage_tile<R, true>(a, blockIdx.x, S, F, wmax, &V);
+    } else {
+        stage_tile<R, false>(a, blockIdx.x, S, F, wmax, nullptr);
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -414,6 +414,123 @@ void rk_update(int rk, int sub, double& u, double& s0, double& s1, double rhs, d
     }
 }
 
+// ---------------------------------------------------------------------------
+// Viscous terms (no live reference counterpart: source/gradient/FourthOrderCS.cpp
+// is never called, Run.cpp:177,188, and its physics is incomplete).  Textbook
+// compressible Navier-Stokes with constant mu and a Prandtl number, T =
+// gamma Ma^2 p / rho (EulerPhase.cpp:137), on the reference's stencil tables
+// (FourthOrderCS.h:27-42) with the interpolation row for i-3/2 corrected to
+// the Lagrange weights [5,15,-5,1]/16.
+
+const double kThetaR[3][6] = {
+    {71.0 / 1920, -141.0 / 128, 69.0 / 64, 1.0 / 192, -3.0 / 128, 3.0 / 640},
+    {-3.0 / 640, 25.0 / 384, -75.0 / 64, 75.0 / 64, -25.0 / 384, 3.0 / 640},
+    {-3.0 / 640, 3.0 / 128, -1.0 / 192, -69.0 / 64, 141.0 / 128, -71.0 / 1920}};
+const double kThetaC[5] = {1.0 / 12, -8.0 / 12, 0.0, 8.0 / 12, -1.0 / 12};
+const double kZetaR[3][4] = {
+    {5.0 / 16, 15.0 / 16, -5.0 / 16, 1.0 / 16},
+    {-1.0 / 16, 9.0 / 16, 9.0 / 16, -1.0 / 16},
+    {1.0 / 16, -5.0 / 16, 15.0 / 16, 5.0 / 16}};
+
+// Any cell index, including corners outside both axes: periodic axes wrap;
+// otherwise the cell is a ghost of the first side found (x before y):
+// Dirichlet = the fixed state, Neumann = the interior cell at the clamped index.
+void cell_any(const Oracle& o, int i, int j, double v[4]) {
+    int side = -1;
+    if (i < 0) { if (o.periodic_x()) i += o.nx; else side = 0; }
+    else if (i >= o.nx) { if (o.periodic_x()) i -= o.nx; else side = 1; }
+    if (j < 0) { if (o.periodic_y()) j += o.ny; else if (side < 0) side = 2; }
+    else if (j >= o.ny) { if (o.periodic_y()) j -= o.ny; else if (side < 0) side = 3; }
+    if (side >= 0 && o.cfg.bc[side] == CFS_BC_DIRICHLET) {
+        for (int q = 0; q < 4; ++q) v[q] = o.cfg.bc_state[side][q];
+        return;
+    }
+    i = i < 0 ? 0 : (i >= o.nx ? o.nx - 1 : i);
+    j = j < 0 ? 0 : (j >= o.ny ? o.ny - 1 : j);
+    for (int q = 0; q < 4; ++q) v[q] = o.U[o.at(q, i, j)];
+}
+
+struct Prim { double u, v, T; };
+
+Prim primitive(const Oracle& o, const double s[4]) {
+    const double ma = o.cfg.ma_ref;
+    const double p = pressure(o.g, s[0], s[1], s[2], s[3]);
+    return {s[1] / s[0], s[2] / s[0], o.g * (ma * ma) * p / s[0]};
+}
+
+Prim prim_any(const Oracle& o, int i, int j) {
+    double s[4];
+    cell_any(o, i, j, s);
+    return primitive(o, s);
+}
+
+double dot_n(const double* c, const double* v, int n) {
+    double s = 0.0;
+    for (int k = 0; k < n; ++k) s += c[k] * v[k];
+    return s;
+}
+
+// 4th-order central cell gradient (FourthOrderCS.cpp:120-151 stencil)
+// of u and v along x (xdir) or y at cell (i, j)
+void cell_grad(const Oracle& o, int i, int j, bool xdir, double& du, double& dv) {
+    double u[5], v[5];
+    for (int k = 0; k < 5; ++k) {
+        const Prim p = xdir ? prim_any(o, i - 2 + k, j) : prim_any(o, i, j - 2 + k);
+        u[k] = p.u;
+        v[k] = p.v;
+    }
+    const double h = xdir ? o.dx : o.dy;
+    du = dot_n(kThetaC, u, 5) / h;
+    dv = dot_n(kThetaC, v, 5) / h;
+}
+
+// Viscous flux through a face with stencil cells idx[0..5] along the line
+// (i = idx, fixed j for x faces; fixed i, j = idx for y faces), written in x
+// form: [0, tau_xx, tau_xy, u tau_xx + v tau_xy + k dT/dx] for x faces and
+// [0, tau_xy, tau_yy, u tau_xy + v tau_yy + k dT/dy] for y faces, each
+// evaluated at i-3/2, i-1/2, i+1/2 and combined with (-1, 26, -1)/24
+// (FourthOrderCS.cpp:77-112).
+void viscous_flux(const Oracle& o, const int idx[6], int line, bool xdir, double out[4]) {
+    const double mu = o.cfg.mu, ma = o.cfg.ma_ref;
+    const double kappa = mu / ((o.g - 1) * (ma * ma) * o.cfg.prandtl);
+    double u[6], v[6], T[6], gu[4], gv[4];
+    for (int k = 0; k < 6; ++k) {
+        const Prim p = xdir ? prim_any(o, idx[k], line) : prim_any(o, line, idx[k]);
+        u[k] = p.u; v[k] = p.v; T[k] = p.T;
+    }
+    for (int m = 0; m < 4; ++m) {   // cross derivatives at stencil cells 1..4
+        if (xdir) cell_grad(o, idx[1 + m], line, false, gu[m], gv[m]);
+        else cell_grad(o, line, idx[1 + m], true, gu[m], gv[m]);
+    }
+    const double h = xdir ? o.dx : o.dy;
+    double F[3][3];
+    for (int r = 0; r < 3; ++r) {
+        const double dudn = dot_n(kThetaR[r], u, 6) / h;
+        const double dvdn = dot_n(kThetaR[r], v, 6) / h;
+        const double dTdn = dot_n(kThetaR[r], T, 6) / h;
+        const double dudt = dot_n(kZetaR[r], gu, 4);
+        const double dvdt = dot_n(kZetaR[r], gv, 4);
+        const double uf = dot_n(kZetaR[r], u + 1, 4);
+        const double vf = dot_n(kZetaR[r], v + 1, 4);
+        if (xdir) {
+            const double txx = mu * ((4.0 / 3.0) * dudn - (2.0 / 3.0) * dvdt);
+            const double txy = mu * (dudt + dvdn);
+            F[r][0] = txx;
+            F[r][1] = txy;
+            F[r][2] = (uf * txx + vf * txy) + kappa * dTdn;
+        } else {
+            const double tyy = mu * ((4.0 / 3.0) * dvdn - (2.0 / 3.0) * dudt);
+            const double txy = mu * (dudn + dvdt);
+            F[r][0] = txy;
+            F[r][1] = tyy;
+            F[r][2] = (uf * txy + vf * tyy) + kappa * dTdn;
+        }
+    }
+    out[0] = 0.0;
+    for (int q = 0; q < 3; ++q)
+        out[q + 1] = ((-1.0 / 24.0) * F[0][q] + (26.0 / 24.0) * F[1][q]) + (-1.0 / 24.0) * F[2][q];
+}
+
 void stage(Oracle& o, int counter, double dt) {
     const int nx = o.nx, ny = o.ny;
     ib_fill(o);
@@ -425,7 +542,13 @@ void stage(Oracle& o, int counter, double dt) {
             for (int f = 0; f <= nx; ++f) {
                 collect_stencil(f, nx, o.periodic_x(), idx);
                 for (int k = 0; k < 6; ++k) st[k] = cell_phase_x(o, idx[k], j);
-                face_flux(o, st, true, &o.FX[((size_t)f * ny + j) * 4]);
+                double* F = &o.FX[((size_t)f * ny + j) * 4];
+                face_flux(o, st, true, F);
+                if (o.cfg.viscous) {   // net flux = inviscid - viscous
+                    double Fv[4];
+                    viscous_flux(o, idx, j, true, Fv);
+                    for (int q = 0; q < 4; ++q) F[q] = F[q] - Fv[q];
+                }
             }
     });
     parallel_for(nx, o.nthreads, [&](int i0, int i1) {
@@ -435,7 +558,13 @@ void stage(Oracle& o, int counter, double dt) {
             for (int gf = 0; gf <= ny; ++gf) {
                 collect_stencil(gf, ny, o.periodic_y(), idx);
                 for (int k = 0; k < 6; ++k) st[k] = cell_phase_y(o, i, idx[k]);
-                face_flux(o, st, false, &o.FY[((size_t)i * (ny + 1) + gf) * 4]);
+                double* F = &o.FY[((size_t)i * (ny + 1) + gf) * 4];
+                face_flux(o, st, false, F);
+                if (o.cfg.viscous) {
+                    double Fv[4];
+                    viscous_flux(o, idx, i, false, Fv);
+                    for (int q = 0; q < 4; ++q) F[q] = F[q] - Fv[q];
+                }
             }
     });
     // RK update of fluid cells (Run.cpp:193-200)
@@ -654,6 +783,14 @@ int oracle_get_rhs(oracle_solver* s, double* R) {
 int oracle_get_face_flux(oracle_solver* s, double* fx, double* fy) {
     std::memcpy(fx, s->o.FX.data(), s->o.FX.size() * sizeof(double));
     std::memcpy(fy, s->o.FY.data(), s->o.FY.size() * sizeof(double));
+    return 0;
+}
+
+// the viscous stencil tables: theta_r [3][6], theta_c [5], zeta_r [3][4]
+int oracle_viscous_tables(double* thr, double* thc, double* zr) {
+    std::memcpy(thr, kThetaR, sizeof(kThetaR));
+    std::memcpy(thc, kThetaC, sizeof(kThetaC));
+    std::memcpy(zr, kZetaR, sizeof(kZetaR));
     return 0;
 }
 

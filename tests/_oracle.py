@@ -33,6 +33,7 @@ def lib():
         getattr(L, n).argtypes = [vp, vp]
         getattr(L, n).restype = ctypes.c_int
     L.oracle_get_face_flux.argtypes = [vp, vp, vp]
+    L.oracle_viscous_tables.argtypes = [vp, vp, vp]
     L.oracle_set_circle.argtypes = [vp, dbl, dbl, dbl, ctypes.c_int, ctypes.c_int]
     L.oracle_set_ib_table.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, vp, vp, vp, vp]
     L.oracle_num_ig.argtypes = [vp]
@@ -151,6 +152,13 @@ class Oracle:
         out = np.zeros(4)
         self.L.oracle_checksums(self.h, _p(out))
         return out
+
+
+def viscous_tables():
+    """(theta_r [3][6], theta_c [5], zeta_r [3][4]) as the oracle holds them."""
+    thr, thc, zr = np.empty((3, 6)), np.empty(5), np.empty((3, 4))
+    lib().oracle_viscous_tables(_p(thr), _p(thc), _p(zr))
+    return thr, thc, zr
 
 
 def weno5(v):

@@ -44,6 +44,10 @@ def test_create_rejects_bad_configs_without_a_gpu():
         cfs.make_config(10, 10, 1.0, 1.0, rk=4),                           # no such RK
         cfs.make_config(10, 10, 1.0, 1.0, bc=(2, 0, 0, 0)),                # periodic on one side only
         cfs.make_config(10, 10, 1.0, 1.0, precision=7),
+        cfs.make_config(10, 10, 1.0, 1.0, viscous=2),
+        cfs.make_config(10, 10, 1.0, 1.0, viscous=1, mu=-1e-3),             # negative viscosity
+        cfs.make_config(10, 10, 1.0, 1.0, viscous=1, mu=1e-3, prandtl=0.0),
+        cfs.make_config(10, 10, 1.0, 1.0, viscous=1, mu=1e-3, ma_ref=0.0),
     ]
     for c in bad:
         assert lib.cfs_create(ctypes.byref(c), None, ctypes.byref(h)) == cfs.CFS_E_ARG
