@@ -34,6 +34,8 @@ CFS_E_OOM = -3
 CFS_E_STATE = -4
 CFS_E_NONFINITE = -5
 CFS_E_UNSUPPORTED = -6
+CFS_E_FULL = -7
+SLOT_FREE, SLOT_RUNNING, SLOT_DONE, SLOT_FAILED = 0, 1, 2, 3
 
 BC_NEUMANN, BC_DIRICHLET, BC_PERIODIC = 0, 1, 2
 SIDE_LEFT, SIDE_RIGHT, SIDE_BOTTOM, SIDE_TOP = 0, 1, 2, 3
@@ -204,6 +206,25 @@ def kelvin_helmholtz(cfg, seed=KH_SEED, amp=0.01, noise=1e-3):
     return np.stack(conserved(rho, u, v, np.full_like(rho, 2.5), gamma=cfg.gamma))
 
 
+class BatchConfig(ctypes.Structure):
+    """cfs_batch_config of include/cfs.h."""
+    _fields_ = [("rk", ctypes.c_int32), ("precision", ctypes.c_int32), ("max_problems", ctypes.c_int32),
+                ("workgroups", ctypes.c_int32), ("arena_bytes", ctypes.c_int64)]
+
+
+class Immersed(ctypes.Structure):
+    """cfs_immersed of include/cfs.h."""
+    _fields_ = [("interp", ctypes.c_int32), ("n_ig", ctypes.c_int32), ("cell_type", ctypes.c_void_p),
+                ("ig_cells", ctypes.c_void_p), ("donors", ctypes.c_void_p), ("image_xy", ctypes.c_void_p),
+                ("coef", ctypes.c_void_p), ("transp", ctypes.c_void_p)]
+
+
+class BatchInfo(ctypes.Structure):
+    """cfs_batch_info of include/cfs.h."""
+    _fields_ = [("slot", ctypes.c_int32), ("state", ctypes.c_int32), ("steps", ctypes.c_int32),
+                ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("t", ctypes.c_double)]
+
+
 # ---------------------------------------------------------------------------
 _LIB = None
 
@@ -238,6 +259,19 @@ def _lib():
     lib.cfs_stage_counter.argtypes = [vp]
     lib.cfs_stage_counter.restype = i32
     lib.cfs_last_kernel_ms.argtypes = [vp, P(dbl)]
+    lib.cfs_batch_create.argtypes = [P(BatchConfig), vp, P(vp)]
+    lib.cfs_batch_destroy.argtypes = [vp]
+    lib.cfs_batch_destroy.restype = None
+    lib.cfs_batch_footprint.argtypes = [P(Config), i32]
+    lib.cfs_batch_footprint.restype = ctypes.c_int64
+    lib.cfs_batch_admit.argtypes = [vp, P(Config), vp, P(Immersed), dbl, dbl, P(i32)]
+    lib.cfs_batch_step.argtypes = [vp, i32]
+    lib.cfs_batch_poll.argtypes = [vp, P(BatchInfo), i32, P(i32)]
+    lib.cfs_batch_retire.argtypes = [vp, i32, vp]
+    lib.cfs_batch_free_bytes.argtypes = [vp, P(ctypes.c_int64)]
+    for name in ("cfs_batch_create", "cfs_batch_admit", "cfs_batch_step", "cfs_batch_poll", "cfs_batch_retire",
+                 "cfs_batch_free_bytes"):
+        getattr(lib, name).restype = i32
     for name in ("cfs_create", "cfs_set_state", "cfs_get_state", "cfs_set_state_device",
                  "cfs_get_state_device", "cfs_circle_setup", "cfs_set_immersed", "cfs_stage",
                  "cfs_step", "cfs_advance", "cfs_compute_dt", "cfs_checksums", "cfs_last_kernel_ms"):
@@ -389,3 +423,114 @@ class Solver:
         ms = ctypes.c_double(0.0)
         _check(self._lib.cfs_last_kernel_ms(self._h, ctypes.byref(ms)), "cfs_last_kernel_ms")
         return ms.value
+
+
+class Batch:
+    """Many independent solves advanced together (a cfs_batch handle).
+
+    admit() returns a slot, or None when the arena or slot table is full;
+    step() queues RK steps of every resident problem; poll() waits and
+    returns {slot: (state, steps, t)}; retire() copies a problem out and
+    frees its slot."""
+
+    def __init__(self, rk=3, precision=PREC_FP64, max_problems=256, arena_bytes=1 << 30, workgroups=0,
+                 stream=None):
+        self._lib = _lib()
+        self.bc = BatchConfig(rk, precision, max_problems, workgroups, arena_bytes)
+        h = ctypes.c_void_p()
+        _check(self._lib.cfs_batch_create(ctypes.byref(self.bc), stream, ctypes.byref(h)), "cfs_batch_create")
+        self._h = h
+        self._cfg = {}
+        self._info = (BatchInfo * max_problems)()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.cfs_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @staticmethod
+    def footprint(cfg, n_ig=0):
+        return int(_lib().cfs_batch_footprint(ctypes.byref(cfg), n_ig))
+
+    def admit(self, cfg, U, cfl=0.5, t_end=0.0, ib=None):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        if U.shape != (4, cfg.nx, cfg.ny):
+            raise ValueError("state shape %s, expected %s" % (U.shape, (4, cfg.nx, cfg.ny)))
+        imm = None
+        keep = [U]
+        if ib is not None:
+            arrs = [np.ascontiguousarray(ib["cell_type"], dtype=np.uint8),
+                    np.ascontiguousarray(ib["ig_cells"], dtype=np.int32),
+                    np.ascontiguousarray(ib["donors"], dtype=np.int32),
+                    np.ascontiguousarray(ib["image_xy"], dtype=np.float64),
+                    np.ascontiguousarray(ib["coef"], dtype=np.float64),
+                    np.ascontiguousarray(ib["transp"], dtype=np.int32)]
+            keep += arrs
+            imm = Immersed(ib["interp"], int(ib["n_ig"]), *[_ptr(a) for a in arrs])
+        slot = ctypes.c_int32(-1)
+        rc = self._lib.cfs_batch_admit(self._h, ctypes.byref(cfg), _ptr(U),
+                                       ctypes.byref(imm) if imm is not None else None, cfl, t_end,
+                                       ctypes.byref(slot))
+        if rc == CFS_E_FULL:
+            return None
+        _check(rc, "cfs_batch_admit")
+        self._cfg[slot.value] = cfg
+        return slot.value
+
+    def step(self, nsteps=1):
+        _check(self._lib.cfs_batch_step(self._h, nsteps), "cfs_batch_step")
+
+    def poll(self):
+        n = ctypes.c_int32(0)
+        _check(self._lib.cfs_batch_poll(self._h, self._info, len(self._info), ctypes.byref(n)), "cfs_batch_poll")
+        return {r.slot: (r.state, r.steps, r.t) for r in self._info[:n.value]}
+
+    def retire(self, slot, copy=True):
+        cfg = self._cfg.pop(slot)
+        U = np.empty((4, cfg.nx, cfg.ny)) if copy else None
+        _check(self._lib.cfs_batch_retire(self._h, slot, _ptr(U) if copy else None), "cfs_batch_retire")
+        return U
+
+    def free_bytes(self):
+        v = ctypes.c_int64(0)
+        _check(self._lib.cfs_batch_free_bytes(self._h, ctypes.byref(v)), "cfs_batch_free_bytes")
+        return v.value
+
+
+# ---------------------------------------------------------------------------
+# batched workload: many two-dimensional Riemann problems of uneven size
+
+def riemann_batch_problems(seed, count=256, sizes=tuple(range(64, 513, 32)), precision=PREC_FP32_BULK, rk=3,
+                           positivity=1, hll_bounds=1):
+    """Seeded random four-quadrant Riemann problems on [0,1]^2 with the split at
+    (0.5, 0.5) and transmissive boundaries: N ~ U(sizes) square, per quadrant
+    rho ~ U[0.1, 2], p ~ logU[0.1, 10], u, v ~ U[-1.5, 1.5].  Returns a list
+    of (cfg, U).  The face-state positivity fallback and velocity HLL bounds
+    are on by default: with the reference's scheme alone most such random
+    states reach a negative pressure within a few steps."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for _ in range(count):
+        n = int(rng.choice(np.asarray(sizes)))
+        states = {}
+        for q in ("NE", "NW", "SW", "SE"):
+            rho = rng.uniform(0.1, 2.0)
+            p = float(np.exp(rng.uniform(np.log(0.1), np.log(10.0))))
+            u, v = rng.uniform(-1.5, 1.5, size=2)
+            states[q] = (rho, float(u), float(v), p)
+        cfg = make_config(n, n, 1.0, 1.0, bc=(BC_NEUMANN,) * 4, rk=rk, precision=precision,
+                          positivity=positivity, hll_bounds=hll_bounds)
+        out.append((cfg, riemann_quadrants(cfg, states)))
+    return out

@@ -41,10 +41,18 @@ def _run(cmd, verbose):
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     objs = []
-    for src in ["cfs_api.hip"]:
+    jobs = []
+    for src in ["cfs_api.hip", "batch.hip"]:   # compiled in parallel
         obj = os.path.join(BUILD, src + ".o")
-        _run([HIPCC, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj] + COMMON + DEVICE, verbose)
+        cmd = [HIPCC, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj] + COMMON + DEVICE
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        jobs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
         objs.append(obj)
+    for cmd, pr in jobs:
+        out, _ = pr.communicate()
+        if pr.returncode != 0:
+            raise RuntimeError("build failed: %s\n%s" % (" ".join(cmd), out))
     for src in ["ib_setup.cpp"]:
         obj = os.path.join(BUILD, src + ".o")
         _run(["g++", "-c", os.path.join(CSRC, src), "-o", obj] + COMMON, verbose)

@@ -490,7 +490,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
         // first stage of a CFL step: every workgroup derives the same dt from
         // the bound accumulated by the previous step's last stage
         const double t = *a.t_in;
-        bool stop = a.t_end > 0.0 && !(t < a.t_end);
+        // a halt is sticky until the host clears it (workgroup 0 rewrites the
+        // flag with the value every workgroup derives, so a read racing with
+        // that write sees the same decision)
+        bool stop = *a.halt != 0 || (a.t_end > 0.0 && !(t < a.t_end));
         bool bad = false;
         double d = 0.0;
         if (!stop) {
@@ -703,6 +706,31 @@ __global__ __launch_bounds__(BLOCK, VISC ? 2 : (sizeof(R) == 8 ? 5 : 6)) void st
     }
 }
 
+// Batched stage: a persistent grid walks a device-resident list of (problem
+// slot, tile) pairs; the arguments of slot p for step parity q and sub-stage s
+// are descs[(p * 2 + q) * 3 + s].  The list is cut into 8 contiguous ranges,
+// one per XCD (workgroup b is dispatched to XCD b % 8), so tiles that share
+// halo cells are staged through the same L2.  Every problem's arithmetic is
+// the single-solve kernel's, so a problem's result does not depend on what
+// else is in the batch.  The grid size is a multiple of 8.
+template <typename R>
+__global__ __launch_bounds__(BLOCK, sizeof(R) == 8 ? 5 : 6) void batch_stage_kernel(
+    const StageArgs* __restrict__ descs, const int2* __restrict__ tiles, const int* __restrict__ ntiles, int parity,
+    int sub) {
+    __shared__ R S[SR * SC * 4];
+    __shared__ R F[(NXF + NYF) * 4];
+    __shared__ double wmax[BLOCK / 64];
+    const long long n = *ntiles;
+    const int per = (int)(gridDim.x >> 3);
+    const int x = (int)(blockIdx.x & 7), l = (int)(blockIdx.x >> 3);
+    const int lo = (int)((n * x) >> 3), hi = (int)((n * (x + 1)) >> 3);
+    for (int t = lo + l; t < hi; t += per) {
+        const int2 pt = tiles[t];
+        stage_tile<R, false>(descs[(pt.x * 2 + parity) * 3 + sub], pt.y, S, F, wmax, nullptr);
+        __syncthreads();   // the next tile reuses S, F and wmax
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Immersed-boundary fill of a whole state (the reference's prologue fill,
 // Run.cpp:106-112); during stages the fill happens in the tile load.
@@ -718,7 +746,7 @@ struct IBArgs {
     const int* transp;
 };
 
-__global__ __launch_bounds__(256) void ib_fill_kernel(IBArgs a) {
+static __global__ __launch_bounds__(256) void ib_fill_kernel(IBArgs a) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= a.n) return;
     double v[4];
@@ -732,7 +760,7 @@ __global__ __launch_bounds__(256) void ib_fill_kernel(IBArgs a) {
 // CFL reduction over a whole state (first step of an advance, cfs_compute_dt):
 // wave64 shuffles, LDS, one atomicMax on the bit pattern (non-negative doubles
 // order as unsigned integers; a NaN wins) -- exact, so order-free.
-__global__ __launch_bounds__(256) void cfl_kernel(const double* __restrict__ U, long long plane,
+static __global__ __launch_bounds__(256) void cfl_kernel(const double* __restrict__ U, long long plane,
                                                   const uint8_t* __restrict__ type, double gamma,
                                                   double dx, double dy, int hll_bounds,
                                                   unsigned long long* acc) {
@@ -748,20 +776,20 @@ __global__ __launch_bounds__(256) void cfl_kernel(const double* __restrict__ U, 
 }
 
 // plain streaming copy (16-byte vectors), the byte-counter calibration kernel
-__global__ __launch_bounds__(256) void copy_kernel(const double2* __restrict__ src, double2* __restrict__ dst,
+static __global__ __launch_bounds__(256) void copy_kernel(const double2* __restrict__ src, double2* __restrict__ dst,
                                                    long long n) {
     for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
          k += (long long)gridDim.x * blockDim.x)
         dst[k] = src[k];
 }
 
-__global__ void set_scalar_kernel(double* p, double v) { *p = v; }
-__global__ void set_u64_kernel(unsigned long long* p, unsigned long long v) { *p = v; }
+static __global__ void set_scalar_kernel(double* p, double v) { *p = v; }
+static __global__ void set_u64_kernel(unsigned long long* p, unsigned long long v) { *p = v; }
 
 // ---------------------------------------------------------------------------
 // Fixed-order checksum tree: chunks of 256 summed by halving in LDS
 // (a[t] += a[t+h], h = 128..1); the chunk sums form the next level.
-__global__ __launch_bounds__(256) void tree_sum_kernel(const double* __restrict__ src, long long n,
+static __global__ __launch_bounds__(256) void tree_sum_kernel(const double* __restrict__ src, long long n,
                                                        long long src_stride, const uint8_t* __restrict__ type,
                                                        double* __restrict__ dst, long long dst_stride) {
     __shared__ double buf[256];

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define CFS_ABI_VERSION 1
+#define CFS_ABI_VERSION 2
 
 typedef enum cfs_status {
     CFS_OK = 0,
@@ -38,7 +38,8 @@ typedef enum cfs_status {
     CFS_E_OOM = -3,         /* device allocation failed */
     CFS_E_STATE = -4,       /* call out of sequence (e.g. stage counter mismatch) */
     CFS_E_NONFINITE = -5,   /* a reduction met NaN/Inf (e.g. negative pressure) */
-    CFS_E_UNSUPPORTED = -6  /* configuration not supported by this build */
+    CFS_E_UNSUPPORTED = -6, /* configuration not supported by this build */
+    CFS_E_FULL = -7         /* batch: no free slot or arena space for the problem now */
 } cfs_status;
 
 /* Physical boundary kinds.  NEUMANN and DIRICHLET follow
@@ -204,6 +205,74 @@ int32_t cfs_set_kernel_timing(cfs_solver* s, int32_t on);
 /* Mean duration (ms) of the stage kernel over the last cfs_advance call made
  * with kernel timing on (0 if none). */
 int32_t cfs_last_kernel_ms(const cfs_solver* s, double* ms_per_stage);
+
+/* ---- batched solves ------------------------------------------------------- */
+/* Many independent problems of uneven sizes advanced together: each call of
+ * cfs_batch_step runs one persistent stage kernel per RK sub-stage over the
+ * tiles of every resident problem (two steps per captured HIP graph).  Each
+ * problem keeps its own CFL time step, end time, boundary conditions and
+ * immersed boundary, and its result is bit-identical to solving it alone
+ * with cfs_advance(nsteps, 0, cfl, t_end).  Problems live in one device arena
+ * sized at creation: admission is by device-memory footprint
+ * (cfs_batch_footprint), and a full arena or slot table is CFS_E_FULL, not an
+ * error of the batch.  No reference counterpart: the reference solves one
+ * problem per run (main.cpp); this is the data-parallel form of its stage
+ * loop (Run.cpp:153-204) for many small solves. */
+typedef struct cfs_batch cfs_batch;
+
+typedef struct cfs_batch_config {
+    int32_t rk;              /* 2 or 3, shared by every problem of the batch */
+    int32_t precision;       /* CFS_PREC_FP64 | CFS_PREC_FP32_BULK, shared */
+    int32_t max_problems;    /* slot table size */
+    int32_t workgroups;      /* persistent grid; 0: CUs x resident workgroups per CU */
+    int64_t arena_bytes;     /* device memory for problem state and tables */
+} cfs_batch_config;
+
+/* Immersed-boundary tables of one problem, as cfs_set_immersed takes them. */
+typedef struct cfs_immersed {
+    int32_t interp;
+    int32_t n_ig;
+    const uint8_t* cell_type;
+    const int32_t* ig_cells;
+    const int32_t* donors;
+    const double* image_xy;
+    const double* coef;
+    const int32_t* transp;
+} cfs_immersed;
+
+enum { CFS_SLOT_FREE = 0, CFS_SLOT_RUNNING = 1, CFS_SLOT_DONE = 2, CFS_SLOT_FAILED = 3 };
+
+typedef struct cfs_batch_info {
+    int32_t slot;
+    int32_t state;           /* CFS_SLOT_* */
+    int32_t steps;           /* RK steps taken */
+    int32_t nx, ny;
+    double t;                /* simulated time reached */
+} cfs_batch_info;
+
+int32_t cfs_batch_create(const cfs_batch_config* bc, void* hip_stream, cfs_batch** out);
+void cfs_batch_destroy(cfs_batch* b);
+/* Arena bytes a problem with n_ig immersed ghosts (0: none) occupies. */
+int64_t cfs_batch_footprint(const cfs_config* cfg, int32_t n_ig);
+/* Admits a problem with initial state U ([4][nx][ny], copied), optional
+ * immersed-boundary tables (NULL: none), CFL number cfl > 0 and end time
+ * t_end (<= 0: runs until retired), starting at t = 0.  The problem's rk and
+ * precision must be the batch's; viscous problems are CFS_E_UNSUPPORTED.
+ * On success *slot names it.  Waits for the batch's queued work. */
+int32_t cfs_batch_admit(cfs_batch* b, const cfs_config* cfg, const double* U, const cfs_immersed* ib,
+                        double cfl, double t_end, int32_t* slot);
+/* Queues nsteps RK steps of every resident problem and returns without
+ * waiting.  Problems past their end time (or with a non-finite bound) stay
+ * halted until retired. */
+int32_t cfs_batch_step(cfs_batch* b, int32_t nsteps);
+/* Waits for the queued work and reports every occupied slot (at most cap
+ * entries into info; *n = number of occupied slots). */
+int32_t cfs_batch_poll(cfs_batch* b, cfs_batch_info* info, int32_t cap, int32_t* n);
+/* Copies a problem's state out (U may be NULL) and frees its slot and arena
+ * space.  Waits for the queued work. */
+int32_t cfs_batch_retire(cfs_batch* b, int32_t slot, double* U);
+/* Free arena bytes (the largest admissible footprint is at most this). */
+int32_t cfs_batch_free_bytes(const cfs_batch* b, int64_t* bytes);
 
 #ifdef __cplusplus
 }

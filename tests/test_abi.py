@@ -27,11 +27,11 @@ def test_every_declared_symbol_is_exported():
 
 def test_version_and_layout():
     lib = cfs.library()
-    assert lib.cfs_abi_version() == 1
+    assert lib.cfs_abi_version() == int(re.search(r"#define CFS_ABI_VERSION (\d+)", open(cfs.HEADER_PATH).read()).group(1))
     lib.cfs_config_size.restype = ctypes.c_int32
     assert lib.cfs_config_size() == ctypes.sizeof(cfs.Config)
-    for st in range(-6, 1):
-        assert lib.cfs_status_string(st)
+    for st in range(-7, 1):
+        assert b"unknown" not in lib.cfs_status_string(st)
     assert cfs.device_count() >= 0
 
 
@@ -70,3 +70,29 @@ def test_circle_setup_argument_errors():
                               one_i.ctypes.data, one_i.ctypes.data, one_d.ctypes.data, one_d.ctypes.data,
                               one_i.ctypes.data)
     assert rc == cfs.CFS_E_ARG and n.value > 1
+
+
+def test_batch_footprint_and_bad_batch_configs_without_a_gpu():
+    lib = cfs.library()
+    cfg = cfs.make_config(100, 60, 1.0, 1.0, rk=3)
+    # 3 FP64 plane-sets, rounded up to 256 B
+    assert cfs.Batch.footprint(cfg) == (96 * 100 * 60 + 255) // 256 * 256
+    with_ib = cfs.Batch.footprint(cfg, n_ig=50)
+    assert with_ib > cfs.Batch.footprint(cfg) + 5 * 100 * 60
+    assert lib.cfs_batch_footprint(ctypes.byref(cfs.make_config(2, 2, 1.0, 1.0)), 0) == -1
+    h = ctypes.c_void_p()
+    for bc in [cfs.BatchConfig(4, 0, 8, 0, 1 << 20), cfs.BatchConfig(3, 5, 8, 0, 1 << 20),
+               cfs.BatchConfig(3, 0, 0, 0, 1 << 20), cfs.BatchConfig(3, 0, 8, -1, 1 << 20),
+               cfs.BatchConfig(3, 0, 8, 0, 0)]:
+        assert lib.cfs_batch_create(ctypes.byref(bc), None, ctypes.byref(h)) == cfs.CFS_E_ARG
+        assert not h.value
+
+
+def test_riemann_batch_problems_are_seeded():
+    a = cfs.riemann_batch_problems(7, count=5)
+    b = cfs.riemann_batch_problems(7, count=5)
+    for (ca, ua), (cb, ub) in zip(a, b):
+        assert ca.nx == cb.nx and np.array_equal(ua, ub)
+    sizes = [c.nx for c, _ in cfs.riemann_batch_problems(1, count=256)]
+    assert min(sizes) >= 64 and max(sizes) <= 512 and all(n % 32 == 0 for n in sizes)
+    assert len(set(sizes)) > 10
