@@ -53,7 +53,7 @@ def build(verbose=False):
         out, _ = pr.communicate()
         if pr.returncode != 0:
             raise RuntimeError("build failed: %s\n%s" % (" ".join(cmd), out))
-    for src in ["ib_setup.cpp"]:
+    for src in ["ib_setup.cpp", "serve.cpp"]:
         obj = os.path.join(BUILD, src + ".o")
         _run(["g++", "-c", os.path.join(CSRC, src), "-o", obj] + COMMON, verbose)
         objs.append(obj)

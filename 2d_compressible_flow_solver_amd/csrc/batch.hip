@@ -278,7 +278,7 @@ int32_t cfs_batch_admit(cfs_batch* b, const cfs_config* cfg, const double* U, co
     if (!(cfl > 0.0) || !(cfl < 1e300) || t_end != t_end) return CFS_E_ARG;
     const bool has_ib = ib != nullptr;
     const int n_ig = has_ib ? ib->n_ig : 0;
-    if (has_ib && n_ig > 0 && (!ib->image_xy || !ib->coef)) return CFS_E_ARG;
+    if (has_ib && (n_ig <= 0 || !ib->image_xy || !ib->coef)) return CFS_E_ARG;
     const int tiles_i = (cfg->nx + TI - 1) / TI, tiles_j = (cfg->ny + TJ - 1) / TJ;
     std::vector<int> index;
     std::vector<uint8_t> tflags;
@@ -448,6 +448,13 @@ int32_t cfs_batch_retire(cfs_batch* b, int32_t slot, double* U) {
     b->alloc.release(s.off, s.bytes);
     s = Slot{};
     b->dirty = true;
+    return CFS_OK;
+}
+
+int32_t cfs_batch_capacity(const cfs_batch* b, int64_t* arena_bytes, int32_t* max_problems) {
+    if (!b || !arena_bytes || !max_problems) return CFS_E_ARG;
+    *arena_bytes = b->bc.arena_bytes;
+    *max_problems = b->bc.max_problems;
     return CFS_OK;
 }
 

@@ -117,6 +117,157 @@ def run_cpu_baseline(budget_s=12.0):
                        "workload, %.1f s, strict-IEEE CPU restatement of the reference" % (steps, 3 * steps, wall))
 
 
+BATCH_SEED = 2024
+BATCH_T_END = 0.2
+
+
+def run_batch(precision, count, poll_every=20, seed=BATCH_SEED, max_steps=20000):
+    """The batched workload: `count` seeded random Riemann problems of uneven
+    size (N ~ U{64, 96, ..., 512}, transmissive boundaries, t_end = 0.2,
+    CFL 0.5, RK3) advanced together through the batch entry points until all
+    reach t_end; then the same problems solved one after another with the
+    single-solve entry points.  Inputs are uploaded at admission (outside the
+    compute but inside the wall time of both legs)."""
+    probs = cfs.riemann_batch_problems(seed, count=count, precision=precision, rk=3)
+    total_cells = sum(c.nx * c.ny for c, _ in probs)
+    arena = int(sum(cfs.Batch.footprint(c) for c, _ in probs) * 1.05) + (64 << 20)
+    res = {}
+    with cfs.Batch(rk=3, precision=precision, max_problems=count, arena_bytes=arena) as b:
+        t0 = time.perf_counter()
+        slot_of = {}
+        for k, (cfg, U) in enumerate(probs):
+            slot_of[b.admit(cfg, U, cfl=CFL, t_end=BATCH_T_END)] = k
+        t_admit = time.perf_counter() - t0
+        polls = 0
+        while slot_of:
+            b.step(poll_every)
+            polls += 1
+            for slot, (state, steps, t) in b.poll().items():
+                if state != cfs.SLOT_RUNNING or steps >= max_steps:
+                    res[slot_of.pop(slot)] = (state, steps, t)
+                    b.retire(slot, copy=False)
+            if polls % 25 == 0:
+                print("batch: %d of %d problems left after %.1f s" % (len(slot_of), count, time.perf_counter() - t0),
+                      file=sys.stderr, flush=True)
+        wall = time.perf_counter() - t0
+    failed = sum(1 for st, _, _ in res.values() if st == cfs.SLOT_FAILED)
+    stalled = sum(1 for st, _, _ in res.values() if st == cfs.SLOT_RUNNING)
+    cell_stages = sum(3 * res[k][1] * probs[k][0].nx * probs[k][0].ny for k in range(count))
+    # the same problems one after another, each alone
+    t0 = time.perf_counter()
+    seq_steps = 0
+    for k, (cfg, U) in enumerate(probs):
+        s = cfs.Solver(cfg)
+        s.set_state(U)
+        try:
+            _, done = s.advance(max_steps, dt=0.0, cfl=CFL, t_end=BATCH_T_END)
+        except cfs.CfsError:
+            done = 0
+        seq_steps += done
+        s.close()
+        if k % 32 == 31:
+            print("one after another: %d of %d solved after %.1f s" % (k + 1, count, time.perf_counter() - t0),
+                  file=sys.stderr, flush=True)
+    seq_wall = time.perf_counter() - t0
+    return dict(problems=count, cells=total_cells, cell_stages=cell_stages, wall=wall, admit_s=t_admit,
+                mcups=cell_stages / wall / 1e6, solves_per_s=count / wall, failed=failed, stalled=stalled,
+                seq_wall=seq_wall, seq_mcups=cell_stages / seq_wall / 1e6, seq_solves_per_s=count / seq_wall,
+                steps_min=min(r[1] for r in res.values()), steps_max=max(r[1] for r in res.values()))
+
+
+def main_batch(args):
+    prec = cfs.PREC_FP64 if args.batch_fp64 else cfs.PREC_FP32_BULK
+    r = run_batch(prec, args.batch_count)
+    print("batch: %d problems, %.2f s batched (%.1f Mcell-updates/s), %.2f s one after another (%.1f)"
+          % (r["problems"], r["wall"], r["mcups"], r["seq_wall"], r["seq_mcups"]), file=sys.stderr, flush=True)
+    out = {
+        "metric": "Mcell-updates/s per RK stage (batched solves)",
+        "value": round(r["mcups"], 2),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f64" if prec == cfs.PREC_FP64 else "f32 (FP64 state)",
+        "data": "synthetic (seeded random four-quadrant Riemann states, seed %d)" % BATCH_SEED,
+        "config": {"workload": "riemann_batch_%d_uneven_N64-512_rk3_cfl0.5_t0.2" % r["problems"],
+                   "precision": "fp64" if prec == cfs.PREC_FP64 else "fp32_bulk",
+                   "total_cells": r["cells"], "positivity": 1, "hll_bounds": 1},
+        "batched": {"wall_s": round(r["wall"], 3), "solves_per_s": round(r["solves_per_s"], 2),
+                    "admission_s": round(r["admit_s"], 3), "failed": r["failed"], "stalled": r["stalled"],
+                    "steps_range": [r["steps_min"], r["steps_max"]]},
+        "one_after_another": {"wall_s": round(r["seq_wall"], 3), "value": round(r["seq_mcups"], 2),
+                              "solves_per_s": round(r["seq_solves_per_s"], 2)},
+        "speedup_vs_one_after_another": round(r["seq_wall"] / r["wall"], 3),
+    }
+    print(json.dumps(out))
+
+
+SERVE_SEED = 7
+
+
+def run_sequential(reqs, precision):
+    """The requests one after another, each alone through the single-solve
+    entry points, in the given precision; returns (wall seconds, failed)."""
+    t0 = time.perf_counter()
+    failed = 0
+    for rq in reqs:
+        cfg = cfs.Config.from_buffer_copy(rq["cfg"])
+        cfg.precision = precision
+        s = cfs.Solver(cfg)
+        if rq["ib"] is not None:
+            s.set_immersed(rq["ib"])
+        s.set_state(rq["U"])
+        try:
+            s.advance(100000, dt=0.0, cfl=rq["cfl"], t_end=rq["t_end"])
+        except cfs.CfsError:
+            failed += 1
+        s.close()
+    return time.perf_counter() - t0, failed
+
+
+def main_serve(args):
+    prec = cfs.PREC_FP64 if args.batch_fp64 else cfs.PREC_FP32_BULK
+    n = args.serve_count
+    arena = args.serve_arena_gb << 30
+    points = []
+    for rate in [float(x) for x in args.serve_rates.split(",")]:
+        burst = rate <= 0
+        reqs = cfs.serving_requests(SERVE_SEED, n, rate=1e9 if burst else rate, precision=prec)
+        if burst:
+            for rq in reqs:
+                rq["arrival_s"] = 0.0
+        with cfs.Batch(rk=3, precision=prec, max_problems=args.serve_slots, arena_bytes=arena) as b:
+            res = b.serve(reqs, steps_per_poll=args.steps_per_poll, keep_states=False)
+        lat = np.array([r["latency_s"] for r in res])
+        span = max(r["finished_s"] for r in res) - min(rq["arrival_s"] for rq in reqs)
+        failed = sum(r["state"] == cfs.SLOT_FAILED for r in res)
+        pt = {"offered_per_s": "burst" if burst else rate, "requests": n, "solves_per_s": round(n / span, 3),
+              "p50_s": round(float(np.percentile(lat, 50)), 4), "p99_s": round(float(np.percentile(lat, 99)), 4),
+              "makespan_s": round(span, 3), "failed": int(failed)}
+        points.append(pt)
+        print("serve: %s" % json.dumps(pt), file=sys.stderr, flush=True)
+    # the same requests one after another in FP64 (and in the serving precision)
+    reqs = cfs.serving_requests(SERVE_SEED, n, rate=1e9, precision=prec)
+    seq64, f64 = run_sequential(reqs, cfs.PREC_FP64)
+    print("one after another, fp64: %.2f s (%d failed)" % (seq64, f64), file=sys.stderr, flush=True)
+    cap = max(p["solves_per_s"] for p in points)
+    out = {
+        "metric": "solves/s (serving, continuous batching)",
+        "value": cap,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f64" if prec == cfs.PREC_FP64 else "f32 (FP64 state)",
+        "data": "synthetic (seeded request stream, seed %d)" % SERVE_SEED,
+        "config": {"workload": "serving_riemann128-512_cylinder256x128-1024x512_rk3_cfl0.5",
+                   "arena_gb": args.serve_arena_gb, "slots": args.serve_slots,
+                   "steps_per_poll": args.steps_per_poll},
+        "points": points,
+        "sequential_fp64": {"wall_s": round(seq64, 3), "solves_per_s": round(n / seq64, 3), "failed": f64},
+        "vs_sequential_fp64": round(cap / (n / seq64), 3),
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,6 +276,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-bulk leg")
     ap.add_argument("--fp32-only", action="store_true", help="time the FP32-bulk leg only (profiling)")
+    ap.add_argument("--workload", choices=["cylinder", "batch", "serve"], default="cylinder",
+                    help="cylinder: the headline line; batch / serve: the batched-solves and serving measurements")
+    ap.add_argument("--serve-count", type=int, default=96)
+    ap.add_argument("--serve-rates", default="10,20,40,0", help="offered request rates per second (0: burst)")
+    ap.add_argument("--serve-arena-gb", type=int, default=2)
+    ap.add_argument("--serve-slots", type=int, default=64)
+    ap.add_argument("--steps-per-poll", type=int, default=5)
+    ap.add_argument("--batch-count", type=int, default=256)
+    ap.add_argument("--batch-fp64", action="store_true", help="batched workload in FP64 instead of FP32-bulk")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="N",
                     help="only run N plain copy-kernel launches over one state plane-set (counter calibration)")
     args = ap.parse_args()
@@ -136,6 +296,12 @@ def main():
     if cfs.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.workload == "batch":
+        main_batch(args)
+        return
+    if args.workload == "serve":
+        main_serve(args)
+        return
     if args.copy_calibration:
         cfg, ib, U = setup(cfs.PREC_FP64)
         s = cfs.Solver(cfg)
@@ -189,7 +355,7 @@ def main():
                    "ib_interpolation": "bilinear_positive", "finite_at_end": r64["finite"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
-                     "kernel": "stage_kernel<double>", "kernel_ms": round(r64["kernel_ms"], 5),
+                     "kernel": "stage_kernel<double, false>", "kernel_ms": round(r64["kernel_ms"], 5),
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,
     }

@@ -252,12 +252,14 @@ typedef struct cfs_batch_info {
 
 int32_t cfs_batch_create(const cfs_batch_config* bc, void* hip_stream, cfs_batch** out);
 void cfs_batch_destroy(cfs_batch* b);
-/* Arena bytes a problem with n_ig immersed ghosts (0: none) occupies. */
+/* Arena bytes a problem occupies: n_ig = 0 without an immersed boundary,
+ * else the number of immersed ghosts of its tables. */
 int64_t cfs_batch_footprint(const cfs_config* cfg, int32_t n_ig);
 /* Admits a problem with initial state U ([4][nx][ny], copied), optional
  * immersed-boundary tables (NULL: none), CFL number cfl > 0 and end time
  * t_end (<= 0: runs until retired), starting at t = 0.  The problem's rk and
- * precision must be the batch's; viscous problems are CFS_E_UNSUPPORTED.
+ * precision must be the batch's; viscous problems are CFS_E_UNSUPPORTED;
+ * immersed-boundary tables need n_ig > 0.
  * On success *slot names it.  Waits for the batch's queued work. */
 int32_t cfs_batch_admit(cfs_batch* b, const cfs_config* cfg, const double* U, const cfs_immersed* ib,
                         double cfl, double t_end, int32_t* slot);
@@ -273,6 +275,36 @@ int32_t cfs_batch_poll(cfs_batch* b, cfs_batch_info* info, int32_t cap, int32_t*
 int32_t cfs_batch_retire(cfs_batch* b, int32_t slot, double* U);
 /* Free arena bytes (the largest admissible footprint is at most this). */
 int32_t cfs_batch_free_bytes(const cfs_batch* b, int64_t* bytes);
+/* Arena size and slot count the batch was created with. */
+int32_t cfs_batch_capacity(const cfs_batch* b, int64_t* arena_bytes, int32_t* max_problems);
+
+/* ---- serving loop --------------------------------------------------------- */
+/* A request: one solve to t_end, arriving at arrival_s seconds after the
+ * start of cfs_serve.  Outputs are written by cfs_serve. */
+typedef struct cfs_request {
+    const cfs_config* cfg;
+    const double* U;          /* initial state [4][nx][ny] */
+    const cfs_immersed* ib;   /* NULL: no immersed boundary */
+    double cfl, t_end;        /* t_end > 0 */
+    double arrival_s;
+    double* U_out;            /* NULL, or [4][nx][ny] for the final state */
+    /* outputs */
+    double admitted_s;        /* admission time (seconds from the start) */
+    double finished_s;        /* time its completion was observed */
+    double t;                 /* simulated time reached */
+    int32_t steps;
+    int32_t state;            /* CFS_SLOT_DONE | CFS_SLOT_FAILED; CFS_SLOT_FREE if never admitted */
+} cfs_request;
+
+/* Runs requests through a batch with continuous batching: requests that have
+ * arrived are admitted in arrival order while their footprint fits the arena
+ * and a slot is free (a request that does not fit waits; later ones that fit
+ * may pass it), every resident problem advances steps_per_poll RK steps per
+ * round, and finished problems are retired at the end of the round they
+ * finish in, freeing their space for waiting requests.  Returns when every
+ * request has finished; requests whose footprint exceeds the whole arena are
+ * CFS_E_FULL before anything runs.  No reference counterpart. */
+int32_t cfs_serve(cfs_batch* b, cfs_request* reqs, int32_t n, int32_t steps_per_poll);
 
 #ifdef __cplusplus
 }
