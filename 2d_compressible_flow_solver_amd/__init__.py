@@ -267,6 +267,8 @@ def _lib():
     lib.cfs_stage_counter.argtypes = [vp]
     lib.cfs_stage_counter.restype = i32
     lib.cfs_last_kernel_ms.argtypes = [vp, P(dbl)]
+    lib.cfs_last_rhs.argtypes = [vp, vp]
+    lib.cfs_last_rhs.restype = i32
     lib.cfs_batch_create.argtypes = [P(BatchConfig), vp, P(vp)]
     lib.cfs_batch_destroy.argtypes = [vp]
     lib.cfs_batch_destroy.restype = None
@@ -428,6 +430,12 @@ class Solver:
         n = ctypes.c_int32(0)
         _check(self._lib.cfs_last_redo_tiles(self._h, ctypes.byref(n)), "cfs_last_redo_tiles")
         return n.value
+
+    def last_rhs(self):
+        """L(U) applied by the last stage on fluid cells, [4][nx][ny]."""
+        R = np.empty((4, self.cfg.nx, self.cfg.ny))
+        _check(self._lib.cfs_last_rhs(self._h, _ptr(R)), "cfs_last_rhs")
+        return R
 
     def last_kernel_ms(self):
         ms = ctypes.c_double(0.0)

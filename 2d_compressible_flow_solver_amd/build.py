@@ -17,6 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcfs.so")
+RUN = os.path.join(HERE, "cfs_run")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -60,6 +61,13 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-o", tmp] + objs + ["--offload-arch=" + ARCH, "-fPIC"], verbose)
     os.replace(tmp, LIB)
+    # the driver: the reference's Run / param.h / plugin roles on the C-ABI
+    drv = os.path.join(HERE, "driver")
+    tmp = RUN + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-Wall", "-o", tmp,
+          os.path.join(drv, "run.cpp"), os.path.join(drv, "main.cpp"),
+          "-L" + HERE, "-lcfs", "-Wl,-rpath,$ORIGIN"], verbose)
+    os.replace(tmp, RUN)
     return LIB
 
 

@@ -39,6 +39,7 @@ struct cfs_solver {
     ProblemDev dev;               // device buffers (planes, types, IB tables, scalars)
     int cur = 0;                  // plane-set holding the current state
     double* tree = nullptr;       // checksum scratch
+    double* rhs = nullptr;        // cfs_last_rhs output (allocated on first use) + 2 ints of counters
     long long tree_len = 0;
     int parity = 0;               // stage parity (exact-range counters)
     int step_parity = 0;          // step parity (time and CFL-bound slots)
@@ -201,6 +202,7 @@ void cfs_destroy(cfs_solver* s) {
     (void)hipFree(s->dev.planes);
     (void)hipFree(s->dev.scal);
     (void)hipFree(s->tree);
+    (void)hipFree(s->rhs);
     (void)hipFree(s->dev.exact);
     if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
@@ -493,6 +495,39 @@ int32_t cfs_last_redo_tiles(cfs_solver* s, int32_t* n) {
 }
 
 int32_t cfs_stage_counter(const cfs_solver* s) { return s ? s->counter : -1; }
+
+int32_t cfs_last_rhs(cfs_solver* s, double* R) {
+    if (!s || !R) return CFS_E_ARG;
+    if (s->counter <= 1) return CFS_E_STATE;
+    const cfs_config& c = s->cfg;
+    const size_t bytes = sizeof(double) * 4 * (size_t)s->dev.plane;
+    if (!s->rhs && hipMalloc(&s->rhs, bytes + 2 * sizeof(int)) != hipSuccess) {
+        s->rhs = nullptr;
+        return CFS_E_OOM;
+    }
+    // the last stage's input plane-set is intact after it (every stage writes
+    // another set): evaluate its RHS again, with its immersed-ghost fill
+    StageArgs a;
+    fill_stage_args(a, s->dev, rk_substage(c.rk, s->counter - 1), s->step_parity, 0, StageTime{});
+    a.form = 3;
+    a.out = s->rhs;
+    a.u0 = a.u1 = nullptr;
+    int* cnt = reinterpret_cast<int*>(s->rhs + 4 * (size_t)s->dev.plane);
+    a.exact_count = cnt;
+    a.exact_other = cnt + 1;
+    const dim3 grid((unsigned)(s->dev.tiles_i * s->dev.tiles_j));
+    if (c.precision == CFS_PREC_FP64) {
+        if (c.viscous) launch_stage_kernel<double, true>(grid, s->stream, a);
+        else launch_stage_kernel<double, false>(grid, s->stream, a);
+    } else {
+        if (c.viscous) launch_stage_kernel<float, true>(grid, s->stream, a);
+        else launch_stage_kernel<float, false>(grid, s->stream, a);
+    }
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(R, s->rhs, bytes, hipMemcpyDeviceToHost, s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    return CFS_OK;
+}
 
 int32_t cfs_last_kernel_ms(const cfs_solver* s, double* ms) {
     if (!s || !ms) return CFS_E_ARG;

@@ -51,6 +51,7 @@ struct StageArgs {
     int nx, ny;
     int tiles_j;          // tiles along j
     int form;             // 0: c1*in + (c2*dt)L   1: (c0*u0 + c1*in) + (c2*dt)L   2: (c0*u0 + c1*u1) + (c2*dt)L
+                          // 3: L itself on fluid cells, 0 elsewhere (no state is written)
     int positivity;       // inadmissible WENO face state -> adjacent cell state
     int hll_bounds;       // 0: reference (momentum, left state in SR), 1: velocity bounds
     int dx_pow2, dy_pow2; // dx (dy) is a power of two: divide by multiplying with 1/dx
@@ -539,7 +540,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
             const int k = a.ig_index[cell];
             ghost_value(a.in, a.plane, a.ib_interp, a.ig_donors + 4 * k, a.ig_coef + 10 * k, a.ig_transp + 4 * k,
                         a.ig_xy + 2 * k, v);
-            if (xs && ys && i0 - HALO + r < a.nx && j0 - HALO + c < a.ny) {   // one of the tile's own cells
+            if (a.form != 3 && xs && ys && i0 - HALO + r < a.nx && j0 - HALO + c < a.ny) {   // own cell
 #pragma unroll
                 for (int q = 0; q < 4; ++q) a.out[q * a.plane + cell] = v[q];
             }
@@ -675,11 +676,14 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
                     const double L = (double)rhs;
                     if (a.form == 0) res[q] = a.c1 * u + c2dt * L;
                     else if (a.form == 1) res[q] = (a.c0 * a.u0[q * a.plane + c] + a.c1 * u) + c2dt * L;
-                    else res[q] = (a.c0 * a.u0[q * a.plane + c] + a.c1 * a.u1[q * a.plane + c]) + c2dt * L;
+                    else if (a.form == 2) res[q] = (a.c0 * a.u0[q * a.plane + c] + a.c1 * a.u1[q * a.plane + c]) + c2dt * L;
+                    else res[q] = L;
+                } else if (a.form == 3) {
+                    res[q] = 0.0;
                 }
             }
             // immersed ghosts were written while staging the tile
-            if (ty != 2) {
+            if (ty != 2 || a.form == 3) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) a.out[q * a.plane + c] = res[q];
             }

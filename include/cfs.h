@@ -196,6 +196,13 @@ int32_t cfs_calibrate_copy(cfs_solver* s, int32_t repeats);
  * division.  Outside it the two differ by at most a few subnormal ulps
  * (absolute error below 1e-320); 0 means the stage was bit-faithful. */
 int32_t cfs_last_redo_tiles(cfs_solver* s, int32_t* n);
+/* The right-hand side L(U) = -(F_{i+1/2} - F_{i-1/2})/dx - (G_{j+1/2} -
+ * G_{j-1/2})/dy that the last stage applied, [4][nx][ny], on fluid cells (0
+ * elsewhere): what the reference keeps per cell from Cell::comRHS
+ * (Cell.cpp:110-125) and writes as FluxRho/FluxMu/FluxMv/FluxEnergy
+ * (Output.cpp:66-69).  Recomputed from the last stage's input, which every
+ * stage leaves intact.  CFS_E_STATE before the first stage. */
+int32_t cfs_last_rhs(cfs_solver* s, double* R);
 /* next stage counter (1 + stages done) */
 int32_t cfs_stage_counter(const cfs_solver* s);
 /* Kernel timing switch (default off): when on, cfs_advance brackets every
