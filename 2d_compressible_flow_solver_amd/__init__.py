@@ -209,7 +209,8 @@ def kelvin_helmholtz(cfg, seed=KH_SEED, amp=0.01, noise=1e-3):
 class BatchConfig(ctypes.Structure):
     """cfs_batch_config of include/cfs.h."""
     _fields_ = [("rk", ctypes.c_int32), ("precision", ctypes.c_int32), ("max_problems", ctypes.c_int32),
-                ("workgroups", ctypes.c_int32), ("arena_bytes", ctypes.c_int64)]
+                ("workgroups", ctypes.c_int32), ("arena_bytes", ctypes.c_int64), ("limiter", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 class Immersed(ctypes.Structure):
@@ -452,9 +453,9 @@ class Batch:
     frees its slot."""
 
     def __init__(self, rk=3, precision=PREC_FP64, max_problems=256, arena_bytes=1 << 30, workgroups=0,
-                 stream=None):
+                 stream=None, limiter=False):
         self._lib = _lib()
-        self.bc = BatchConfig(rk, precision, max_problems, workgroups, arena_bytes)
+        self.bc = BatchConfig(rk, precision, max_problems, workgroups, arena_bytes, 1 if limiter else 0, 0)
         h = ctypes.c_void_p()
         _check(self._lib.cfs_batch_create(ctypes.byref(self.bc), stream, ctypes.byref(h)), "cfs_batch_create")
         self._h = h

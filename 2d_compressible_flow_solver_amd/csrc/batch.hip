@@ -141,18 +141,21 @@ struct cfs_batch {
 
 namespace {
 
-template <typename R>
+template <typename R, bool LIMIT>
 int launch_batch_stage(cfs_batch* b, int parity, int sub) {
-    hipLaunchKernelGGL(batch_stage_kernel<R>, dim3((unsigned)b->grid), dim3(BLOCK), 0, b->stream, b->descs, b->tiles,
-                       b->ntiles, parity, sub);
+    hipLaunchKernelGGL((batch_stage_kernel<R, LIMIT>), dim3((unsigned)b->grid), dim3(BLOCK), 0, b->stream, b->descs,
+                       b->tiles, b->ntiles, parity, sub);
     HIP_OK(hipGetLastError());
     return CFS_OK;
 }
 
 int launch_batch_step(cfs_batch* b, int parity) {
+    const bool fp64 = b->bc.precision == CFS_PREC_FP64, lim = b->bc.limiter != 0;
     for (int k = 0; k < b->bc.rk; ++k) {
-        const int rc = (b->bc.precision == CFS_PREC_FP64) ? launch_batch_stage<double>(b, parity, k)
-                                                          : launch_batch_stage<float>(b, parity, k);
+        const int rc = fp64 ? (lim ? launch_batch_stage<double, true>(b, parity, k)
+                                   : launch_batch_stage<double, false>(b, parity, k))
+                            : (lim ? launch_batch_stage<float, true>(b, parity, k)
+                                   : launch_batch_stage<float, false>(b, parity, k));
         if (rc) return rc;
     }
     return CFS_OK;
@@ -201,6 +204,7 @@ int32_t cfs_batch_create(const cfs_batch_config* bc, void* hip_stream, cfs_batch
     if (bc->precision != CFS_PREC_FP64 && bc->precision != CFS_PREC_FP32_BULK) return CFS_E_ARG;
     if (bc->max_problems < 1 || bc->max_problems > (1 << 20) || bc->arena_bytes < 1 || bc->workgroups < 0)
         return CFS_E_ARG;
+    if (bc->limiter != 0 && bc->limiter != 1) return CFS_E_ARG;
     cfs_batch* b = new (std::nothrow) cfs_batch();
     if (!b) return CFS_E_OOM;
     b->bc = *bc;
@@ -215,10 +219,13 @@ int32_t cfs_batch_create(const cfs_batch_config* bc, void* hip_stream, cfs_batch
     int dev = 0, cus = 0, per_cu = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (e == hipSuccess)
-        e = (bc->precision == CFS_PREC_FP64)
-                ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_stage_kernel<double>, BLOCK, 0)
-                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_stage_kernel<float>, BLOCK, 0);
+    if (e == hipSuccess) {
+        const bool fp64 = bc->precision == CFS_PREC_FP64, lim = bc->limiter != 0;
+        e = fp64 ? (lim ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_stage_kernel<double, true>, BLOCK, 0)
+                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_stage_kernel<double, false>, BLOCK, 0))
+                 : (lim ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_stage_kernel<float, true>, BLOCK, 0)
+                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_stage_kernel<float, false>, BLOCK, 0));
+    }
     if (e != hipSuccess) { cfs_batch_destroy(b); return CFS_E_HIP; }
     int g = bc->workgroups > 0 ? bc->workgroups : cus * (per_cu > 0 ? per_cu : 1);
     b->grid = ((g + 7) / 8) * 8;
@@ -275,6 +282,7 @@ int32_t cfs_batch_admit(cfs_batch* b, const cfs_config* cfg, const double* U, co
     if (rc) return rc;
     if (cfg->rk != b->bc.rk || cfg->precision != b->bc.precision) return CFS_E_ARG;
     if (cfg->viscous) return CFS_E_UNSUPPORTED;
+    if (cfg->positivity == 2 && !b->bc.limiter) return CFS_E_ARG;   // the batch runs the kernel without the limiter
     if (!(cfl > 0.0) || !(cfl < 1e300) || t_end != t_end) return CFS_E_ARG;
     const bool has_ib = ib != nullptr;
     const int n_ig = has_ib ? ib->n_ig : 0;

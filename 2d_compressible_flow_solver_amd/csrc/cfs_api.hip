@@ -64,9 +64,23 @@ void drop_graph(cfs_solver* s) {
     s->graph_key = GraphKey{};
 }
 
-template <typename R, bool VISC>
+template <typename R, bool VISC, bool LIMIT>
 void launch_stage_kernel(dim3 grid, hipStream_t stream, const StageArgs& a) {
-    hipLaunchKernelGGL((stage_kernel<R, VISC>), grid, dim3(BLOCK), 0, stream, a);
+    hipLaunchKernelGGL((stage_kernel<R, VISC, LIMIT>), grid, dim3(BLOCK), 0, stream, a);
+}
+
+// the kernel variant of a configuration: precision, viscous terms, limiter
+void dispatch_stage(const cfs_config& c, dim3 grid, hipStream_t stream, const StageArgs& a) {
+    const bool limit = c.positivity == 2;
+    if (c.precision == CFS_PREC_FP64) {
+        if (c.viscous) launch_stage_kernel<double, true, false>(grid, stream, a);
+        else if (limit) launch_stage_kernel<double, false, true>(grid, stream, a);
+        else launch_stage_kernel<double, false, false>(grid, stream, a);
+    } else {
+        if (c.viscous) launch_stage_kernel<float, true, false>(grid, stream, a);
+        else if (limit) launch_stage_kernel<float, false, true>(grid, stream, a);
+        else launch_stage_kernel<float, false, false>(grid, stream, a);
+    }
 }
 
 // One RK stage of the global stage counter.
@@ -83,13 +97,7 @@ int launch_stage(cfs_solver* s, int counter, bool timed, const StageTime& tm) {
         s->ev.push_back(e1);
         HIP_OK(hipEventRecord(e0, s->stream));
     }
-    if (c.precision == CFS_PREC_FP64) {
-        if (c.viscous) launch_stage_kernel<double, true>(grid, s->stream, a);
-        else launch_stage_kernel<double, false>(grid, s->stream, a);
-    } else {
-        if (c.viscous) launch_stage_kernel<float, true>(grid, s->stream, a);
-        else launch_stage_kernel<float, false>(grid, s->stream, a);
-    }
+    dispatch_stage(c, grid, s->stream, a);
     HIP_OK(hipGetLastError());
     if (timed) HIP_OK(hipEventRecord(s->ev.back(), s->stream));
     s->parity ^= 1;
@@ -516,13 +524,7 @@ int32_t cfs_last_rhs(cfs_solver* s, double* R) {
     a.exact_count = cnt;
     a.exact_other = cnt + 1;
     const dim3 grid((unsigned)(s->dev.tiles_i * s->dev.tiles_j));
-    if (c.precision == CFS_PREC_FP64) {
-        if (c.viscous) launch_stage_kernel<double, true>(grid, s->stream, a);
-        else launch_stage_kernel<double, false>(grid, s->stream, a);
-    } else {
-        if (c.viscous) launch_stage_kernel<float, true>(grid, s->stream, a);
-        else launch_stage_kernel<float, false>(grid, s->stream, a);
-    }
+    dispatch_stage(c, grid, s->stream, a);
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(R, s->rhs, bytes, hipMemcpyDeviceToHost, s->stream));
     HIP_OK(hipStreamSynchronize(s->stream));

@@ -37,12 +37,13 @@ inline int check_config(const cfs_config* c) {
     if ((c->bc[2] == CFS_BC_PERIODIC) != (c->bc[3] == CFS_BC_PERIODIC)) return CFS_E_ARG;
     if (c->rk != 2 && c->rk != 3) return CFS_E_ARG;
     if (c->precision != CFS_PREC_FP64 && c->precision != CFS_PREC_FP32_BULK) return CFS_E_ARG;
-    if (c->positivity != 0 && c->positivity != 1) return CFS_E_ARG;
+    if (c->positivity < 0 || c->positivity > 2) return CFS_E_ARG;
     if (c->hll_bounds != 0 && c->hll_bounds != 1) return CFS_E_ARG;
     if (c->viscous != 0 && c->viscous != 1) return CFS_E_ARG;
     if (c->viscous) {
         if (!(c->mu >= 0) || !(c->mu < 1e300) || !(c->prandtl > 0) || !(c->ma_ref > 0)) return CFS_E_ARG;
         if (!(c->prandtl < 1e300) || !(c->ma_ref < 1e150)) return CFS_E_ARG;
+        if (c->positivity == 2) return CFS_E_UNSUPPORTED;   // the limiter covers the inviscid flux only
     }
     return CFS_OK;
 }

@@ -28,6 +28,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace cfs {
 
 constexpr int TI = 15;             // tile rows (x / i direction)
@@ -403,6 +405,112 @@ __device__ __forceinline__ void block_max_to(double m, unsigned long long* acc, 
 // cells.  Tiles holding a value outside the range where the WENO5 constant
 // divisions are bit-identical to IEEE division are counted in exact_count.
 // ---------------------------------------------------------------------------
+// Positivity-preserving flux limiter (positivity = 2; no reference
+// counterpart).  Hu, Adams & Shu, J. Comput. Phys. 242 (2013) 169-180: each
+// face flux is blended with the Lax-Friedrichs flux of the two adjacent cell
+// states, F = F_LF + theta (F_hat - F_LF), theta in [0, 1] the largest value
+// for which both cells' shares of the forward-Euler update stay admissible
+// (rho, p >= min(1e-13, their Lax-Friedrichs value)).  A cell's update
+// U - ln (F_r - F_l) - lt (G_t - G_b) (ln = dt/h_n) is split into one share per
+// face, U -+ k (F_face - F(U)) with k = 2 (ln an + lt at) / an, an (at) the
+// largest |u_n| + c (|u_t| + c) over the cell and its two neighbours along n
+// (t); the Lax-Friedrichs shares are admissible when ln an + lt at <= 1/2,
+// which the CFL bound gives for CFL <= 1/4 and, in practice, at 1/2.  With
+// SSP Runge-Kutta stages (RK2, and RK3 except the reference's last stage) the
+// new state keeps rho, p > 0.  All in the face-normal frame.
+
+template <typename R> struct LimitLDS {
+    R sx[SR * SC], sy[SR * SC];   // |u| + c and |v| + c of every staged cell
+};
+
+template <typename R> __device__ __forceinline__ R maxr(R x, R y) { return (x < y) ? y : x; }
+
+template <typename R>
+__device__ __forceinline__ R pressure_n(R r, R n, R t, R E, R gm1) {
+    return gm1 * (E - K<R>::div(K<R>::half * (n * n + t * t), r));
+}
+
+// largest theta in [0, 1] keeping B0 + theta*Dl admissible: density first,
+// then pressure, which is concave along the density-limited segment
+template <typename R>
+__device__ __forceinline__ R share_theta(const R B0[4], const R Dl[4], R gm1) {
+    using C = K<R>;
+    const R eps = R(1e-13);
+    const R r0 = B0[0];
+    const R p0 = pressure_n<R>(B0[0], B0[1], B0[2], B0[3], gm1);
+    const R er = (r0 < eps) ? r0 : eps;
+    const R ep = (p0 < eps) ? p0 : eps;
+    R th = R(1);
+    const R r1 = B0[0] + Dl[0];
+    if (r1 < er) th = C::div(r0 - er, r0 - r1);
+    R B1[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) B1[q] = B0[q] + th * Dl[q];
+    const R p1 = pressure_n<R>(B1[0], B1[1], B1[2], B1[3], gm1);
+    if (p1 < ep) th = th * C::div(p0 - ep, p0 - p1);
+    return th;
+}
+
+template <typename R>
+__device__ __forceinline__ void euler_flux_n(R r, R n, R t, R E, R gm1, R F[4]) {
+    using C = K<R>;
+    const R p = pressure_n<R>(r, n, t, E, gm1);
+    const R un = C::div(n, r);
+    F[0] = n;
+    F[1] = C::div(n * n, r) + p;
+    F[2] = C::div(n * t, r);
+    F[3] = (E + p) * un;
+}
+
+// cL, cR: LDS cells left and right of the face; dn, dt: LDS offsets of the
+// neighbours along the normal and the tangent; ln, lt: dt/h along them
+template <typename R>
+__device__ __forceinline__ void limit_face(const R* __restrict__ S, const LimitLDS<R>& Lm, int cL, int cR, int nrm,
+                                           int tng, int dn, int dtg, R ln, R lt, R g, R fl[4]) {
+    using C = K<R>;
+    const R* sn = (nrm == 1) ? Lm.sx : Lm.sy;
+    const R* st = (nrm == 1) ? Lm.sy : Lm.sx;
+    const R gm1 = g - R(1);
+    const R UL[4] = {S[cL * 4], S[cL * 4 + nrm], S[cL * 4 + tng], S[cL * 4 + 3]};
+    const R UR[4] = {S[cR * 4], S[cR * 4 + nrm], S[cR * 4 + tng], S[cR * 4 + 3]};
+    R FL[4], FR[4];
+    euler_flux_n<R>(UL[0], UL[1], UL[2], UL[3], gm1, FL);
+    euler_flux_n<R>(UR[0], UR[1], UR[2], UR[3], gm1, FR);
+    const R a = maxr<R>(sn[cL], sn[cR]);
+    R Flf[4], D[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        Flf[q] = C::half * (FL[q] + FR[q]) - (C::half * a) * (UR[q] - UL[q]);
+        D[q] = fl[q] - Flf[q];
+    }
+    const R anL = maxr<R>(maxr<R>(sn[cL - dn], sn[cL]), sn[cL + dn]);
+    const R atL = maxr<R>(maxr<R>(st[cL - dtg], st[cL]), st[cL + dtg]);
+    const R anR = maxr<R>(maxr<R>(sn[cR - dn], sn[cR]), sn[cR + dn]);
+    const R atR = maxr<R>(maxr<R>(st[cR - dtg], st[cR]), st[cR + dtg]);
+    const R kL = C::div(R(2) * (ln * anL + lt * atL), anL);
+    const R kR = C::div(R(2) * (ln * anR + lt * atR), anR);
+    R B0[4], Dl[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        B0[q] = UL[q] - kL * (Flf[q] - FL[q]);
+        Dl[q] = -(kL * D[q]);
+    }
+    const R thL = share_theta<R>(B0, Dl, gm1);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        B0[q] = UR[q] + kR * (Flf[q] - FR[q]);
+        Dl[q] = kR * D[q];
+    }
+    const R thR = share_theta<R>(B0, Dl, gm1);
+    R th = (thR < thL) ? thR : thL;
+    th = (th < R(0)) ? R(0) : th;
+    if (th < R(1)) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) fl[q] = Flf[q] + th * D[q];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Viscous terms: textbook compressible Navier-Stokes (constant mu, Prandtl
 // number, T = gamma Ma^2 p/rho as EulerPhase.cpp:137) on the stencil tables
 // of source/gradient/FourthOrderCS.h:27-42 (whose own viscous code is never
@@ -475,9 +583,9 @@ __device__ __forceinline__ void viscous_flux(const ViscLDS<R>& V, const int cs[6
         out[q] = ((R(-1.0) / R(24.0)) * acc[0][q] + (R(26.0) / R(24.0)) * acc[1][q]) + (R(-1.0) / R(24.0)) * acc[2][q];
 }
 
-template <typename R, bool VISC>
+template <typename R, bool VISC, bool LIMIT>
 __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R* F, double* wmax,
-                                           ViscLDS<R>* V) {
+                                           ViscLDS<R>* V, LimitLDS<R>* Lm) {
     const int tid = threadIdx.x;
     const int ti = tile / a.tiles_j;
     const int tj = tile - ti * a.tiles_j;
@@ -532,7 +640,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
     for (int s = tid; s < SR * SC; s += BLOCK) {
         const int r = s / SC, c = s - r * SC;
         const bool xs = (c >= HALO && c < HALO + TJ), ys = (r >= HALO && r < HALO + TI);
-        if (!VISC && !(xs || ys)) continue;   // the viscous cross terms read the corners
+        if (!VISC && !LIMIT && !(xs || ys)) continue;   // viscous cross terms and the limiter read the corners
         double v[4];
         const long long cell = fetch_cell(a, i0 - HALO + r, j0 - HALO + c, v);
         if (near_ib && cell >= 0 && a.type[cell] == 2) {
@@ -554,6 +662,22 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
     // one barrier + OR over the tile: count tiles outside the range where the
     // constant-division shortcut is bit-identical to IEEE division
     if (__syncthreads_or(unsafe) && tid == 0) atomicAdd(a.exact_count, 1);
+
+    if constexpr (LIMIT) {
+        if (a.positivity == 2) {
+            // |u| + c and |v| + c of every staged cell
+            const R g = (R)a.gamma, gm1 = g - R(1);
+            for (int s = tid; s < SR * SC; s += BLOCK) {
+                const R rho = S[s * 4], mx = S[s * 4 + 1], my = S[s * 4 + 2], E = S[s * 4 + 3];
+                const R p = gm1 * (E - K<R>::div(K<R>::half * (mx * mx + my * my), rho));
+                const R c = K<R>::sqrt_(K<R>::div(g * p, rho));
+                const R ux = K<R>::div(mx, rho), uy = K<R>::div(my, rho);
+                Lm->sx[s] = (ux < R(0) ? -ux : ux) + c;
+                Lm->sy[s] = (uy < R(0) ? -uy : uy) + c;
+            }
+            __syncthreads();
+        }
+    }
 
     if constexpr (VISC) {
         // primitive variables of every staged cell (EulerPhase.cpp:130-137 forms)
@@ -597,6 +721,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
     // ---- faces ----
     const bool px = a.bc[0] == BC_PERIODIC, py = a.bc[2] == BC_PERIODIC;
     const R g = (R)a.gamma;
+    // Lax-Friedrichs shares of the limiter: dt/dx and dt/dy
+    const R lam_x = (R)(dt / a.dx), lam_y = (R)(dt / a.dy);
     for (int s = tid; s < NXF + NYF; s += BLOCK) {
         int cell[6], nrm, tng;
         if (s < NXF) {
@@ -627,6 +753,13 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
         }
         R fl[4];
         face_flux<R>(S, cell, nrm, tng, g, a.positivity, a.hll_bounds, fl);
+        if constexpr (LIMIT) {
+            if (a.positivity == 2) {
+                const bool xf = s < NXF;
+                limit_face<R>(S, *Lm, cell[2] >> 2, cell[3] >> 2, nrm, tng, xf ? SC : 1, xf ? 1 : SC,
+                              xf ? lam_x : lam_y, xf ? lam_y : lam_x, g, fl);
+            }
+        }
         if constexpr (VISC) {   // net flux = inviscid - viscous (mass: minus zero)
             int cs[6];
 #pragma unroll
@@ -697,16 +830,20 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
 // The stage kernel: one workgroup per tile.  FP64: at most 96 VGPRs so that
 // five workgroups (20 waves) share a CU; FP32: six (the most without spills).
 // The viscous variant's LDS (57 KB in FP64) allows two workgroups per CU.
-template <typename R, bool VISC>
-__global__ __launch_bounds__(BLOCK, VISC ? 2 : (sizeof(R) == 8 ? 5 : 6)) void stage_kernel(StageArgs a) {
+// The limiter variant adds 7.4 KB (FP64) of LDS: four per CU.
+template <typename R, bool VISC, bool LIMIT>
+__global__ __launch_bounds__(BLOCK, VISC ? 2 : (LIMIT ? 4 : (sizeof(R) == 8 ? 5 : 6))) void stage_kernel(StageArgs a) {
     __shared__ R S[SR * SC * 4];
     __shared__ R F[(NXF + NYF) * 4];
     __shared__ double wmax[BLOCK / 64];
     if constexpr (VISC) {
         __shared__ ViscLDS<R> V;
-        stage_tile<R, true>(a, blockIdx.x, S, F, wmax, &V);
+        stage_tile<R, true, false>(a, blockIdx.x, S, F, wmax, &V, nullptr);
+    } else if constexpr (LIMIT) {
+        __shared__ LimitLDS<R> Lm;
+        stage_tile<R, false, true>(a, blockIdx.x, S, F, wmax, nullptr, &Lm);
     } else {
-        stage_tile<R, false>(a, blockIdx.x, S, F, wmax, nullptr);
+        stage_tile<R, false, false>(a, blockIdx.x, S, F, wmax, nullptr, nullptr);
     }
 }
 
@@ -717,20 +854,25 @@ __global__ __launch_bounds__(BLOCK, VISC ? 2 : (sizeof(R) == 8 ? 5 : 6)) void st
 // halo cells are staged through the same L2.  Every problem's arithmetic is
 // the single-solve kernel's, so a problem's result does not depend on what
 // else is in the batch.  The grid size is a multiple of 8.
-template <typename R>
-__global__ __launch_bounds__(BLOCK, sizeof(R) == 8 ? 5 : 6) void batch_stage_kernel(
+template <typename R, bool LIMIT>
+__global__ __launch_bounds__(BLOCK, LIMIT ? 4 : (sizeof(R) == 8 ? 5 : 6)) void batch_stage_kernel(
     const StageArgs* __restrict__ descs, const int2* __restrict__ tiles, const int* __restrict__ ntiles, int parity,
     int sub) {
     __shared__ R S[SR * SC * 4];
     __shared__ R F[(NXF + NYF) * 4];
     __shared__ double wmax[BLOCK / 64];
+    using LimitT = typename std::conditional<LIMIT, LimitLDS<R>, char>::type;
+    __shared__ LimitT Lm;
     const long long n = *ntiles;
     const int per = (int)(gridDim.x >> 3);
     const int x = (int)(blockIdx.x & 7), l = (int)(blockIdx.x >> 3);
     const int lo = (int)((n * x) >> 3), hi = (int)((n * (x + 1)) >> 3);
     for (int t = lo + l; t < hi; t += per) {
         const int2 pt = tiles[t];
-        stage_tile<R, false>(descs[(pt.x * 2 + parity) * 3 + sub], pt.y, S, F, wmax, nullptr);
+        if constexpr (LIMIT)
+            stage_tile<R, false, true>(descs[(pt.x * 2 + parity) * 3 + sub], pt.y, S, F, wmax, nullptr, &Lm);
+        else
+            stage_tile<R, false, false>(descs[(pt.x * 2 + parity) * 3 + sub], pt.y, S, F, wmax, nullptr, nullptr);
         __syncthreads();   // the next tile reuses S, F and wmax
     }
 }

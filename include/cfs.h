@@ -84,7 +84,11 @@ typedef struct cfs_config {
                                 reconstruction is not admissible (rho > 0 and p > 0 fail)
                                 is replaced by the adjacent cell's state (first order);
                                 the reference turns such a face into NaN through
-                                sqrt(gamma*p/rho) whenever p/rho < 0. */
+                                sqrt(gamma*p/rho) whenever p/rho < 0.  2 = as 1, plus
+                                the positivity-preserving flux limiter (each face flux
+                                blended with the Lax-Friedrichs flux so that the
+                                forward-Euler update keeps rho, p > 0; Hu, Adams & Shu
+                                2013); inviscid configurations only. */
     int32_t hll_bounds;      /* 0 = reference HLL outer wave-speed bounds
                                 SL = min(mL - aL, u* - a*L), SR = max(mL + aL, u* + a*R)
                                 with m the normal MOMENTUM and the left state in SR
@@ -233,6 +237,8 @@ typedef struct cfs_batch_config {
     int32_t max_problems;    /* slot table size */
     int32_t workgroups;      /* persistent grid; 0: CUs x resident workgroups per CU */
     int64_t arena_bytes;     /* device memory for problem state and tables */
+    int32_t limiter;         /* 1: kernel with the positivity limiter (problems may use positivity = 2) */
+    int32_t reserved;        /* 0 */
 } cfs_batch_config;
 
 /* Immersed-boundary tables of one problem, as cfs_set_immersed takes them. */

@@ -531,6 +531,105 @@ void viscous_flux(const Oracle& o, const int idx[6], int line, bool xdir, double
         out[q + 1] = ((-1.0 / 24.0) * F[0][q] + (26.0 / 24.0) * F[1][q]) + (-1.0 / 24.0) * F[2][q];
 }
 
+// ---------------------------------------------------------------------------
+// Positivity-preserving flux limiter (positivity = 2; no reference
+// counterpart; Hu, Adams & Shu, J. Comput. Phys. 242 (2013) 169-180), in the
+// face-normal frame (n: normal momentum, t: tangential): F = F_LF +
+// theta (F_hat - F_LF), theta the largest value in [0, 1] keeping both
+// adjacent cells' shares U -+ k (F - F(U)) admissible, k = 2 (ln an + lt at)/an.
+
+double maxr(double x, double y) { return (x < y) ? y : x; }
+
+double pressure_n(double g, double r, double n, double t, double E) {
+    return (g - 1) * (E - 0.5 * (n * n + t * t) / r);
+}
+
+void euler_flux_n(double g, const double U[4], double F[4]) {
+    const double p = pressure_n(g, U[0], U[1], U[2], U[3]);
+    const double un = U[1] / U[0];
+    F[0] = U[1];
+    F[1] = U[1] * U[1] / U[0] + p;
+    F[2] = U[1] * U[2] / U[0];
+    F[3] = (U[3] + p) * un;
+}
+
+// |u| + c (xdir) or |v| + c of any cell, ghosts included
+double wave_speed(const Oracle& o, int i, int j, bool xdir) {
+    double s[4];
+    cell_any(o, i, j, s);
+    const double p = pressure(o.g, s[0], s[1], s[2], s[3]);
+    const double c = std::sqrt(o.g * p / s[0]);
+    const double u = (xdir ? s[1] : s[2]) / s[0];
+    return (u < 0.0 ? -u : u) + c;
+}
+
+double share_theta(double g, const double B0[4], const double Dl[4]) {
+    const double eps = 1e-13;
+    const double r0 = B0[0];
+    const double p0 = pressure_n(g, B0[0], B0[1], B0[2], B0[3]);
+    const double er = (r0 < eps) ? r0 : eps;
+    const double ep = (p0 < eps) ? p0 : eps;
+    double th = 1.0;
+    const double r1 = B0[0] + Dl[0];
+    if (r1 < er) th = (r0 - er) / (r0 - r1);
+    double B1[4];
+    for (int q = 0; q < 4; ++q) B1[q] = B0[q] + th * Dl[q];
+    const double p1 = pressure_n(g, B1[0], B1[1], B1[2], B1[3]);
+    if (p1 < ep) th = th * ((p0 - ep) / (p0 - p1));
+    return th;
+}
+
+// face between cells (iL, jL) and (iR, jR) (stencil entries 2 and 3), normal
+// along x (xdir) or y; F in the x frame, limited in place
+void limit_face(const Oracle& o, const Phase& cL, const Phase& cR, int iL, int jL, int iR, int jR, bool xdir,
+                double dt, double F[4]) {
+    const double g = o.g;
+    const int nrm = xdir ? 1 : 2, tng = xdir ? 2 : 1;
+    const double UL[4] = {cL.rho, xdir ? cL.mu : cL.mv, xdir ? cL.mv : cL.mu, cL.E};
+    const double UR[4] = {cR.rho, xdir ? cR.mu : cR.mv, xdir ? cR.mv : cR.mu, cR.E};
+    const double fh[4] = {F[0], F[nrm], F[tng], F[3]};
+    double FL[4], FR[4];
+    euler_flux_n(g, UL, FL);
+    euler_flux_n(g, UR, FR);
+    const double a = maxr(wave_speed(o, iL, jL, xdir), wave_speed(o, iR, jR, xdir));
+    double Flf[4], D[4];
+    for (int q = 0; q < 4; ++q) {
+        Flf[q] = 0.5 * (FL[q] + FR[q]) - (0.5 * a) * (UR[q] - UL[q]);
+        D[q] = fh[q] - Flf[q];
+    }
+    const int di = xdir ? 1 : 0, dj = xdir ? 0 : 1;   // neighbour offsets along the normal
+    auto share_k = [&](int i, int j) {
+        const double an = maxr(maxr(wave_speed(o, i - di, j - dj, xdir), wave_speed(o, i, j, xdir)),
+                               wave_speed(o, i + di, j + dj, xdir));
+        const double at = maxr(maxr(wave_speed(o, i - dj, j - di, !xdir), wave_speed(o, i, j, !xdir)),
+                               wave_speed(o, i + dj, j + di, !xdir));
+        const double ln = dt / (xdir ? o.dx : o.dy), lt = dt / (xdir ? o.dy : o.dx);
+        return 2.0 * (ln * an + lt * at) / an;
+    };
+    const double kL = share_k(iL, jL), kR = share_k(iR, jR);
+    double B0[4], Dl[4];
+    for (int q = 0; q < 4; ++q) {
+        B0[q] = UL[q] - kL * (Flf[q] - FL[q]);
+        Dl[q] = -(kL * D[q]);
+    }
+    const double thL = share_theta(g, B0, Dl);
+    for (int q = 0; q < 4; ++q) {
+        B0[q] = UR[q] + kR * (Flf[q] - FR[q]);
+        Dl[q] = kR * D[q];
+    }
+    const double thR = share_theta(g, B0, Dl);
+    double th = (thR < thL) ? thR : thL;
+    th = (th < 0.0) ? 0.0 : th;
+    if (th < 1.0) {
+        double fn[4];
+        for (int q = 0; q < 4; ++q) fn[q] = Flf[q] + th * D[q];
+        F[0] = fn[0];
+        F[nrm] = fn[1];
+        F[tng] = fn[2];
+        F[3] = fn[3];
+    }
+}
+
 void stage(Oracle& o, int counter, double dt) {
     const int nx = o.nx, ny = o.ny;
     ib_fill(o);
@@ -544,6 +643,7 @@ void stage(Oracle& o, int counter, double dt) {
                 for (int k = 0; k < 6; ++k) st[k] = cell_phase_x(o, idx[k], j);
                 double* F = &o.FX[((size_t)f * ny + j) * 4];
                 face_flux(o, st, true, F);
+                if (o.cfg.positivity == 2) limit_face(o, st[2], st[3], idx[2], j, idx[3], j, true, dt, F);
                 if (o.cfg.viscous) {   // net flux = inviscid - viscous
                     double Fv[4];
                     viscous_flux(o, idx, j, true, Fv);
@@ -560,6 +660,7 @@ void stage(Oracle& o, int counter, double dt) {
                 for (int k = 0; k < 6; ++k) st[k] = cell_phase_y(o, i, idx[k]);
                 double* F = &o.FY[((size_t)i * (ny + 1) + gf) * 4];
                 face_flux(o, st, false, F);
+                if (o.cfg.positivity == 2) limit_face(o, st[2], st[3], i, idx[2], i, idx[3], false, dt, F);
                 if (o.cfg.viscous) {
                     double Fv[4];
                     viscous_flux(o, idx, i, false, Fv);

@@ -13,15 +13,16 @@ pytestmark = pytest.mark.gpu
 SUBSONIC = (1.0, 1.0, 0.0, 1.0)
 
 
-def _problems(precision, rk):
+def _problems(precision, rk, positivity=1):
     probs = []
     for k, (cfg, U) in enumerate(cfs.riemann_batch_problems(11, count=6, sizes=(37, 48, 64, 75, 96),
-                                                            precision=precision, rk=rk)):
+                                                            precision=precision, rk=rk, positivity=positivity)):
         probs.append((cfg, U, None, 0.5, 0.01 + 0.004 * k))
-    cyl = cfs.config_cylinder(96, 48, 4.0, 2.0, freestream=SUBSONIC, positivity=0, hll_bounds=0, rk=rk,
-                              precision=precision)
+    fs = SUBSONIC if positivity < 2 else (1.4, 3.0, 0.0, 1.0)
+    cyl = cfs.config_cylinder(96, 48, 4.0, 2.0, freestream=fs, positivity=positivity if positivity == 2 else 0,
+                              hll_bounds=0, rk=rk, precision=precision)
     ib = cfs.circle_setup(cyl, 1.0, 1.0, 0.25, interp=cfs.IB_BILINEAR_POSITIVE)
-    probs.append((cyl, cfs.cylinder_freestream(cyl, ib["cell_type"], freestream=SUBSONIC), ib, 0.45, 0.15))
+    probs.append((cyl, cfs.cylinder_freestream(cyl, ib["cell_type"], freestream=fs), ib, 0.45, 0.15))
     return probs
 
 
@@ -60,10 +61,13 @@ def _run_batch(b, probs, first):
     return out
 
 
-@pytest.mark.parametrize("precision,rk", [(cfs.PREC_FP64, 3), (cfs.PREC_FP32_BULK, 3), (cfs.PREC_FP64, 2)])
-def test_batch_equals_alone(precision, rk):
-    probs = _problems(precision, rk)
-    with cfs.Batch(rk=rk, precision=precision, max_problems=16, arena_bytes=256 << 20) as b:
+@pytest.mark.parametrize("precision,rk,positivity", [(cfs.PREC_FP64, 3, 1), (cfs.PREC_FP32_BULK, 3, 1),
+                                                     (cfs.PREC_FP64, 2, 1), (cfs.PREC_FP64, 3, 2),
+                                                     (cfs.PREC_FP32_BULK, 2, 2)])
+def test_batch_equals_alone(precision, rk, positivity):
+    probs = _problems(precision, rk, positivity)
+    with cfs.Batch(rk=rk, precision=precision, max_problems=16, arena_bytes=256 << 20,
+                   limiter=positivity == 2) as b:
         got = _run_batch(b, probs, first=3)
     for k, (cfg, U, ib, cfl, t_end) in enumerate(probs):
         ref, t, done = _alone(cfg, U, ib, cfl, t_end)

@@ -349,3 +349,49 @@ def test_viscous_fp32_bulk_l1_density():
     l1 = np.abs(s32.get_state()[0] - s64.get_state()[0]).mean()
     print("viscous FP32-bulk L1(rho) after 100 steps:", l1)
     assert l1 <= 1e-4
+
+
+# ---------------------------------------------------------------------------
+# positivity-preserving flux limiter (positivity = 2)
+
+@pytest.mark.parametrize("rk", [2, 3])
+def test_limiter_random_riemann(rk):
+    for cfg, U in cfs.riemann_batch_problems(2024, count=3, sizes=(48, 67), precision=cfs.PREC_FP64, rk=rk,
+                                             positivity=2):
+        s, o = _pair(cfg, U)
+        t, done = s.advance(40, dt=0.0, cfl=0.5)
+        for n in range(40):
+            o.step(0.5 / o.cfl_smax())
+        assert done == 40
+        _assert_parity(s.get_state(), o.get_state(), "limiter rk%d %dx%d" % (rk, cfg.nx, cfg.ny))
+
+
+def test_limiter_mach3_cylinder():
+    fs = (1.4, 3.0, 0.0, 1.0)
+    cfg = cfs.config_cylinder(128, 64, 4.0, 2.0, freestream=fs, positivity=2, hll_bounds=0)
+    ib = cfs.circle_setup(cfg, 1.0, 1.0, 0.125, interp=cfs.IB_BILINEAR_POSITIVE)
+    U = cfs.cylinder_freestream(cfg, ib["cell_type"], freestream=fs)
+    s, o = _pair(cfg, U, ib)
+    t, done = s.advance(60, dt=0.0, cfl=0.5)
+    for n in range(60):
+        o.step(0.5 / o.cfl_smax())
+    assert done == 60
+    _assert_parity(s.get_state(), o.get_state(), "limiter Mach-3 cylinder")
+
+
+def test_limiter_fp32_bulk_l1_density():
+    cfg64 = cfs.config_riemann(64, rk=3)
+    cfg32 = cfs.config_riemann(64, rk=3, precision=cfs.PREC_FP32_BULK)
+    cfg64.positivity = cfg32.positivity = 2
+    U = cfs.riemann_quadrants(cfg64)
+    o = O.Oracle(cfg64, nthreads=4)
+    o.set_state(U)
+    s = cfs.Solver(cfg32)
+    s.set_state(U)
+    dt = 0.5 / o.cfl_smax()
+    for n in range(100):
+        o.step(dt)
+        s.step(dt)
+    l1 = np.abs(s.get_state()[0] - o.get_state()[0]).mean()
+    print("limiter FP32-bulk L1(rho) after 100 steps:", l1)
+    assert l1 <= 1e-4
