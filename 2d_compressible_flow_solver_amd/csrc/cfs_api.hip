@@ -88,7 +88,7 @@ int launch_stage(cfs_solver* s, int counter, bool timed, const StageTime& tm) {
     const cfs_config& c = s->cfg;
     StageArgs a;
     const int next = fill_stage_args(a, s->dev, rk_substage(c.rk, counter), s->step_parity, s->parity, tm);
-    const dim3 grid((unsigned)(s->dev.tiles_i * s->dev.tiles_j));
+    const dim3 grid((unsigned)(8 * ((s->dev.tiles_i * s->dev.tiles_j + 7) / 8)));
     if (timed) {
         hipEvent_t e0, e1;
         HIP_OK(hipEventCreate(&e0));
@@ -523,7 +523,7 @@ int32_t cfs_last_rhs(cfs_solver* s, double* R) {
     int* cnt = reinterpret_cast<int*>(s->rhs + 4 * (size_t)s->dev.plane);
     a.exact_count = cnt;
     a.exact_other = cnt + 1;
-    const dim3 grid((unsigned)(s->dev.tiles_i * s->dev.tiles_j));
+    const dim3 grid((unsigned)(8 * ((s->dev.tiles_i * s->dev.tiles_j + 7) / 8)));
     dispatch_stage(c, grid, s->stream, a);
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(R, s->rhs, bytes, hipMemcpyDeviceToHost, s->stream));

@@ -111,6 +111,7 @@ inline int fill_stage_args(StageArgs& a, const ProblemDev& p, int sub, int step_
     a.nx = c.nx;
     a.ny = c.ny;
     a.tiles_j = p.tiles_j;
+    a.ntiles = p.tiles_i * p.tiles_j;
     a.dx = c.lx / c.nx;
     a.dy = c.ly / c.ny;
     a.inv_dx = 1.0 / a.dx;

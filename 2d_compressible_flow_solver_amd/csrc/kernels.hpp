@@ -52,6 +52,7 @@ struct StageArgs {
     long long plane;      // nx*ny
     int nx, ny;
     int tiles_j;          // tiles along j
+    int ntiles;           // tiles of the grid
     int form;             // 0: c1*in + (c2*dt)L   1: (c0*u0 + c1*in) + (c2*dt)L   2: (c0*u0 + c1*u1) + (c2*dt)L
                           // 3: L itself on fluid cells, 0 elsewhere (no state is written)
     int positivity;       // inadmissible WENO face state -> adjacent cell state
@@ -435,11 +436,13 @@ __device__ __forceinline__ R pressure_n(R r, R n, R t, R E, R gm1) {
 template <typename R>
 __device__ __forceinline__ R share_theta(const R B0[4], const R Dl[4], R gm1) {
     using C = K<R>;
-    const R eps = R(1e-13);
     const R r0 = B0[0];
     const R p0 = pressure_n<R>(B0[0], B0[1], B0[2], B0[3], gm1);
-    const R er = (r0 < eps) ? r0 : eps;
-    const R ep = (p0 < eps) ? p0 : eps;
+    // floors: min(1e-13, Lax-Friedrichs value) in FP64; in FP32 a thousandth of
+    // the Lax-Friedrichs value, which stays clear of the rounding of FP32 fluxes
+    const R eps = R(1e-13);
+    const R er = (sizeof(R) == 8) ? ((r0 < eps) ? r0 : eps) : r0 * R(1e-3);
+    const R ep = (sizeof(R) == 8) ? ((p0 < eps) ? p0 : eps) : p0 * R(1e-3);
     R th = R(1);
     const R r1 = B0[0] + Dl[0];
     if (r1 < er) th = C::div(r0 - er, r0 - r1);
@@ -451,14 +454,17 @@ __device__ __forceinline__ R share_theta(const R B0[4], const R Dl[4], R gm1) {
     return th;
 }
 
+// Euler flux of a cell state in the face-normal frame, one division: with
+// ir = 1/r, un = n*ir, ut = t*ir, p = gm1*(E - half*(n*un + t*ut))
 template <typename R>
 __device__ __forceinline__ void euler_flux_n(R r, R n, R t, R E, R gm1, R F[4]) {
     using C = K<R>;
-    const R p = pressure_n<R>(r, n, t, E, gm1);
-    const R un = C::div(n, r);
+    const R ir = C::div(R(1), r);
+    const R un = n * ir, ut = t * ir;
+    const R p = gm1 * (E - C::half * (n * un + t * ut));
     F[0] = n;
-    F[1] = C::div(n * n, r) + p;
-    F[2] = C::div(n * t, r);
+    F[1] = n * un + p;
+    F[2] = n * ut;
     F[3] = (E + p) * un;
 }
 
@@ -669,9 +675,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
             const R g = (R)a.gamma, gm1 = g - R(1);
             for (int s = tid; s < SR * SC; s += BLOCK) {
                 const R rho = S[s * 4], mx = S[s * 4 + 1], my = S[s * 4 + 2], E = S[s * 4 + 3];
-                const R p = gm1 * (E - K<R>::div(K<R>::half * (mx * mx + my * my), rho));
-                const R c = K<R>::sqrt_(K<R>::div(g * p, rho));
-                const R ux = K<R>::div(mx, rho), uy = K<R>::div(my, rho);
+                const R ir = K<R>::div(R(1), rho);
+                const R ux = mx * ir, uy = my * ir;
+                const R p = gm1 * (E - K<R>::half * (mx * ux + my * uy));
+                const R c = K<R>::sqrt_((g * p) * ir);
                 Lm->sx[s] = (ux < R(0) ? -ux : ux) + c;
                 Lm->sy[s] = (uy < R(0) ? -uy : uy) + c;
             }
@@ -831,19 +838,28 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
 // five workgroups (20 waves) share a CU; FP32: six (the most without spills).
 // The viscous variant's LDS (57 KB in FP64) allows two workgroups per CU.
 // The limiter variant adds 7.4 KB (FP64) of LDS: four per CU.
+//
+// Tile order: workgroups are dispatched round-robin over the 8 XCDs
+// (workgroup b to XCD b % 8), each with its own L2.  The grid is 8 x ceil(n/8)
+// and workgroup b takes tile (b % 8) * ceil(n/8) + b / 8, so each XCD sweeps
+// one contiguous band of tiles and neighbouring tiles, which share halo
+// cells, are staged through the same L2.
 template <typename R, bool VISC, bool LIMIT>
 __global__ __launch_bounds__(BLOCK, VISC ? 2 : (LIMIT ? 4 : (sizeof(R) == 8 ? 5 : 6))) void stage_kernel(StageArgs a) {
     __shared__ R S[SR * SC * 4];
     __shared__ R F[(NXF + NYF) * 4];
     __shared__ double wmax[BLOCK / 64];
+    const int chunk = (a.ntiles + 7) >> 3;
+    const int tile = (int)(blockIdx.x & 7) * chunk + (int)(blockIdx.x >> 3);
+    if (tile >= a.ntiles) return;
     if constexpr (VISC) {
         __shared__ ViscLDS<R> V;
-        stage_tile<R, true, false>(a, blockIdx.x, S, F, wmax, &V, nullptr);
+        stage_tile<R, true, false>(a, tile, S, F, wmax, &V, nullptr);
     } else if constexpr (LIMIT) {
         __shared__ LimitLDS<R> Lm;
-        stage_tile<R, false, true>(a, blockIdx.x, S, F, wmax, nullptr, &Lm);
+        stage_tile<R, false, true>(a, tile, S, F, wmax, nullptr, &Lm);
     } else {
-        stage_tile<R, false, false>(a, blockIdx.x, S, F, wmax, nullptr, nullptr);
+        stage_tile<R, false, false>(a, tile, S, F, wmax, nullptr, nullptr);
     }
 }
 

@@ -1,23 +1,26 @@
 """Benchmark of the per-RK-stage hot path on one MI355X.
 
-Workload (the headline configuration that fits one GPU): inviscid flow past
-a circular cylinder with the ghost-cell immersed boundary, 2048 x 1024 cells
-on [0,4] x [0,2], r = 0.125 at (1,1), 400 Lagrangian points, 3 ghost layers,
-Dirichlet inflow on the left and extrapolation (Neumann) elsewhere,
-gamma = 1.4, TVD-RK3, CFL 0.5, FP64, the reference's scheme unchanged
-(reference HLL bounds, no positivity fallback).  Inflow: the reference's own
-cylinder freestream rho = 1, u = 1, p = 1 (Cylinder.cpp:5-16): with the
-reference's scheme a Mach-3 inflow turns non-finite within the first 100
-steps at this resolution, and this inflow within ~800, so the timed window is
-520 steps (20 warm-up + 500).  The image-point interpolation is the
-positivity-preserving bilinear one (the reference's LDLT-on-lower-triangle
-interpolation gives negative ghost densities on this grid at step 1).
-One bench "step" = one RK3 step (3 stages: IB fill + fused
-WENO5/HLL/RK stage kernel each, plus the device CFL reduction).
+Workload (the headline configuration, one GPU): Mach-3 inviscid flow past a
+circular cylinder with the ghost-cell immersed boundary, 2048 x 1024 cells on
+[0,4] x [0,2], r = 0.125 at (1,1), 400 Lagrangian points, 3 ghost layers;
+freestream rho = 1.4, u = 3, v = 0, p = 1 (c = 1); Dirichlet (supersonic)
+inflow on the left, extrapolation (Neumann) elsewhere; gamma = 1.4, TVD-RK3,
+CFL 0.5, 2000 timed RK3 steps (after 20 warm-up steps), FP64, then again in
+FP32-bulk.  The reference's scheme goes non-finite within ~10 steps on this
+flow (a near-vacuum wake behind the impulsively started body), so the
+positivity-preserving flux limiter is on (positivity = 2: WENO face-state
+fallback + Lax-Friedrichs blending per face); HLL keeps the reference's
+bounds.  The image-point interpolation is the positivity-preserving bilinear
+one (the reference's LDLT-on-lower-triangle interpolation gives negative
+ghost densities).  One bench "step" = one RK3 step: 3 fused stage kernels
+(IB fill in the tile load, WENO5/HLL/limiter/RK update, CFL bound in the last
+stage, dt in the first), two steps per HIP graph replay.
 
 Metric: Mcell-updates/s per RK stage = cells * stages / wall / 1e6, with the
-state resident in HBM.  The FP32-bulk form (FP64 state, FP32 reconstruction /
-flux / RHS) is timed on the same workload and reported beside it.
+state resident in HBM.  Beside it: the FP32-bulk form (FP64 state, FP32
+reconstruction / flux / RHS) on the same workload, and the reference's scheme
+unchanged (no limiter) on the same grid and body with the reference's own
+subsonic inflow rho = u = p = 1 (Cylinder.cpp:5-16), 500 steps.
 
 roofline: the stage kernel, algorithmic bytes per launch = cells * 97 B
 (RK3 mean 96 B/cell-stage of FP64 state planes + 1 B cell-type mask) over its
@@ -26,6 +29,7 @@ cpu_baseline: the strict-IEEE CPU restatement of the reference (oracle/) on
 the same workload, all host threads (up to 16), a bounded number of steps.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+       python bench.py --workload batch | serve    (batched / serving measurements)
 """
 import argparse
 import importlib
@@ -44,7 +48,8 @@ HBM_PEAK_GBS = 8000.0
 NX, NY, LX, LY = 2048, 1024, 4.0, 2.0
 CX, CY, RADIUS = 1.0, 1.0, 0.125
 CFL = 0.5
-FREESTREAM = (1.0, 1.0, 0.0, 1.0)
+FREESTREAM = (1.4, 3.0, 0.0, 1.0)           # Mach 3 at c = 1
+REFERENCE_INFLOW = (1.0, 1.0, 0.0, 1.0)     # Cylinder.cpp:5-16
 BYTES_PER_CELL_STAGE = 97.0
 
 
@@ -52,7 +57,7 @@ def traffic_from_profiles():
     """HBM bytes per stage-kernel launch measured with rocprofv3 --pmc
     (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), from the
     committed summary; None when absent."""
-    p = os.path.join(ROOT, "profiles", "stage_kernel_fp64_cylinder_2048x1024_traffic.txt")
+    p = os.path.join(ROOT, "profiles", "stage_kernel_limiter_fp64_cylinder_mach3_2048x1024_traffic.txt")
     if not os.path.exists(p):
         return None
     for line in open(p):
@@ -61,16 +66,17 @@ def traffic_from_profiles():
     return None
 
 
-def setup(precision):
-    cfg = cfs.config_cylinder(NX, NY, LX, LY, freestream=FREESTREAM, precision=precision, positivity=0,
-                              hll_bounds=0)
+def setup(precision, reference_scheme=False):
+    fs = REFERENCE_INFLOW if reference_scheme else FREESTREAM
+    cfg = cfs.config_cylinder(NX, NY, LX, LY, freestream=fs, precision=precision,
+                              positivity=0 if reference_scheme else 2, hll_bounds=0)
     ib = cfs.circle_setup(cfg, CX, CY, RADIUS, 400, 3, interp=cfs.IB_BILINEAR_POSITIVE)
-    U = cfs.cylinder_freestream(cfg, ib["cell_type"], freestream=FREESTREAM)
+    U = cfs.cylinder_freestream(cfg, ib["cell_type"], freestream=fs)
     return cfg, ib, U
 
 
-def run_gpu(precision, steps, warmup):
-    cfg, ib, U = setup(precision)
+def run_gpu(precision, steps, warmup, reference_scheme=False):
+    cfg, ib, U = setup(precision, reference_scheme)
     s = cfs.Solver(cfg)
     s.set_immersed(ib)
     s.set_state(U)
@@ -271,10 +277,11 @@ def main_serve(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-bulk leg")
+    ap.add_argument("--no-reference-scheme", action="store_true", help="skip the no-limiter leg")
     ap.add_argument("--fp32-only", action="store_true", help="time the FP32-bulk leg only (profiling)")
     ap.add_argument("--workload", choices=["cylinder", "batch", "serve"], default="cylinder",
                     help="cylinder: the headline line; batch / serve: the batched-solves and serving measurements")
@@ -325,6 +332,11 @@ def main():
     if not args.no_fp32:
         r32 = run_gpu(cfs.PREC_FP32_BULK, args.steps, args.warmup)
         print("fp32-bulk: %.1f Mcell-updates/s" % r32["mcups"], file=sys.stderr, flush=True)
+    rref = None
+    if not args.no_reference_scheme:
+        rref = run_gpu(cfs.PREC_FP64, min(args.steps, 500), args.warmup, reference_scheme=True)
+        print("reference scheme (no limiter, subsonic inflow): %.1f Mcell-updates/s, stage kernel %.4f ms"
+              % (rref["mcups"], rref["kernel_ms"]), file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         cpu = run_cpu_baseline()
@@ -346,19 +358,26 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (uniform freestream rho=1, u=1, p=1 on fluid cells; no seed needed)",
-        "config": {"workload": "cylinder_ib_2048x1024_rk3_cfl0.5_fp64", "nx": NX, "ny": NY,
-                   "inflow": {"rho": 1.0, "u": 1.0, "v": 0.0, "p": 1.0},
+        "data": "synthetic (uniform Mach-3 freestream rho=1.4, u=3, p=1 on fluid cells; no seed needed)",
+        "config": {"workload": "cylinder_ib_mach3_2048x1024_rk3_cfl0.5_fp64", "nx": NX, "ny": NY,
+                   "inflow": {"rho": FREESTREAM[0], "u": FREESTREAM[1], "v": FREESTREAM[2], "p": FREESTREAM[3]},
+                   "positivity_limiter": True, "hll_bounds": "reference",
                    "domain": [LX, LY], "cylinder": {"centre": [CX, CY], "radius": RADIUS,
                                                     "lagrangian_points": 400, "ghost_layers": 3},
                    "rk": 3, "cfl": CFL, "precision": "fp64", "stages_per_step": 3,
                    "ib_interpolation": "bilinear_positive", "finite_at_end": r64["finite"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
-                     "kernel": "stage_kernel<double, false>", "kernel_ms": round(r64["kernel_ms"], 5),
+                     "kernel": "stage_kernel<double, false, true>", "kernel_ms": round(r64["kernel_ms"], 5),
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,
     }
+    if rref is not None:
+        out["reference_scheme"] = {"value": round(rref["mcups"], 2), "unit": "Mcell-updates/s",
+                                   "kernel": "stage_kernel<double, false, false>",
+                                   "kernel_ms": round(rref["kernel_ms"], 5),
+                                   "workload": "same grid and body, inflow rho=u=p=1, no limiter, %d steps"
+                                               % rref["steps"]}
     if r32 is not None:
         out["fp32_bulk"] = {"value": round(r32["mcups"], 2), "unit": "Mcell-updates/s",
                             "ms_per_step": round(r32["ms_per_step"], 4),

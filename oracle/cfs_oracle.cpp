@@ -544,22 +544,27 @@ double pressure_n(double g, double r, double n, double t, double E) {
     return (g - 1) * (E - 0.5 * (n * n + t * t) / r);
 }
 
+// one division: ir = 1/r, un = n*ir, ut = t*ir, p = (g-1)*(E - 0.5*(n*un + t*ut))
 void euler_flux_n(double g, const double U[4], double F[4]) {
-    const double p = pressure_n(g, U[0], U[1], U[2], U[3]);
-    const double un = U[1] / U[0];
+    const double ir = 1.0 / U[0];
+    const double un = U[1] * ir, ut = U[2] * ir;
+    const double p = (g - 1) * (U[3] - 0.5 * (U[1] * un + U[2] * ut));
     F[0] = U[1];
-    F[1] = U[1] * U[1] / U[0] + p;
-    F[2] = U[1] * U[2] / U[0];
+    F[1] = U[1] * un + p;
+    F[2] = U[1] * ut;
     F[3] = (U[3] + p) * un;
 }
 
-// |u| + c (xdir) or |v| + c of any cell, ghosts included
+// |u| + c (xdir) or |v| + c of any cell, ghosts included; ir = 1/rho,
+// p = (g-1)*(E - 0.5*(mx*ux + my*uy)), c = sqrt((g*p)*ir)
 double wave_speed(const Oracle& o, int i, int j, bool xdir) {
     double s[4];
     cell_any(o, i, j, s);
-    const double p = pressure(o.g, s[0], s[1], s[2], s[3]);
-    const double c = std::sqrt(o.g * p / s[0]);
-    const double u = (xdir ? s[1] : s[2]) / s[0];
+    const double ir = 1.0 / s[0];
+    const double ux = s[1] * ir, uy = s[2] * ir;
+    const double p = (o.g - 1) * (s[3] - 0.5 * (s[1] * ux + s[2] * uy));
+    const double c = std::sqrt((o.g * p) * ir);
+    const double u = xdir ? ux : uy;
     return (u < 0.0 ? -u : u) + c;
 }
 
