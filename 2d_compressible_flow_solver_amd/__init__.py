@@ -571,16 +571,19 @@ class Batch:
 # ---------------------------------------------------------------------------
 # batched workload: many two-dimensional Riemann problems of uneven size
 
-def riemann_batch_problems(seed, count=256, sizes=tuple(range(64, 513, 32)), precision=PREC_FP32_BULK, rk=3,
-                           positivity=1, hll_bounds=1):
+def riemann_batch_problems(seed, count=256, sizes=tuple(range(64, 513, 32)), precision=PREC_FP32_BULK, rk=2,
+                           positivity=2, hll_bounds=1):
     """Seeded random four-quadrant Riemann problems on [0,1]^2 with the split at
     (0.5, 0.5) and transmissive boundaries: N ~ U(sizes) square, per quadrant
     rho ~ U[0.1, 2], p ~ logU[0.1, 10], u, v ~ U[-1.5, 1.5].  Returns a list
-    of (cfg, U).  The face-state positivity fallback and velocity HLL bounds
-    are on by default: with the reference's scheme alone about 40% of such
-    random states reach a negative pressure within a few steps; with them
-    about 10% still do, where four rarefactions meet at the quadrant corner
-    (the scheme has no positivity-preserving limiter)."""
+    of (cfg, U).  The positivity-preserving limiter (positivity = 2) and
+    velocity HLL bounds are on by default: with the reference's scheme alone
+    about 40% of such random states reach a negative pressure within a few
+    steps, with the face-state fallback alone (positivity = 1) about 30% at
+    N = 64-512 (four rarefactions meeting at the quadrant corner), with the
+    limiter none -- with TVD-RK2 (the reference driver's scheme, Run.cpp:47);
+    the reference's RK3 as written (last stage not a convex combination of
+    forward-Euler steps) still loses positivity on some of them."""
     rng = np.random.Generator(np.random.PCG64(seed))
     out = []
     for _ in range(count):
@@ -597,16 +600,17 @@ def riemann_batch_problems(seed, count=256, sizes=tuple(range(64, 513, 32)), pre
     return out
 
 
-def serving_requests(seed, count, rate, precision=PREC_FP32_BULK, rk=3, riemann_sizes=tuple(range(128, 513, 64)),
-                     cylinder_ny=tuple(range(128, 513, 64)), mach=(0.5, 1.5), radius=(0.05, 0.2),
-                     t_end_riemann=(0.05, 0.2), t_end_cylinder=(0.1, 0.5)):
+def serving_requests(seed, count, rate, precision=PREC_FP32_BULK, rk=2, riemann_sizes=tuple(range(128, 1025, 64)),
+                     cylinder_ny=tuple(range(128, 513, 64)), mach=(1.5, 4.0), radius=(0.05, 0.2),
+                     t_end_riemann=(0.1, 0.5), t_end_cylinder=(0.1, 0.5)):
     """Seeded request stream: Poisson arrivals at `rate` per second; half
     four-quadrant Riemann problems drawn as in riemann_batch_problems (N x N,
     N ~ U(riemann_sizes), t_end ~ U(t_end_riemann)), half cylinders on
     [0,4] x [0,2] (2N x N, N ~ U(cylinder_ny), inflow rho = 1.4, p = 1, Mach ~
     U(mach), radius ~ U(radius) at (1, 1), t_end ~ U(t_end_cylinder)).  The
-    face-state positivity fallback and velocity HLL bounds are on.  Returns a
-    list of request dicts (cfg, U, ib, cfl, t_end, arrival_s)."""
+    positivity-preserving limiter is on (batches serving them need
+    limiter=True).  Returns a list of request dicts (cfg, U, ib, cfl, t_end,
+    arrival_s)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     t = 0.0
     reqs = []
@@ -623,7 +627,7 @@ def serving_requests(seed, count, rate, precision=PREC_FP32_BULK, rk=3, riemann_
             r = float(rng.uniform(*radius))
             fs = (1.4, m, 0.0, 1.0)
             cfg = config_cylinder(2 * ny, ny, 4.0, 2.0, freestream=fs, rk=rk, precision=precision,
-                                  positivity=1, hll_bounds=1)
+                                  positivity=2, hll_bounds=1)
             ib = circle_setup(cfg, 1.0, 1.0, r, 400, 3, interp=IB_BILINEAR_POSITIVE)
             reqs.append(dict(cfg=cfg, U=cylinder_freestream(cfg, ib["cell_type"], freestream=fs), ib=ib, cfl=0.5,
                              t_end=float(rng.uniform(*t_end_cylinder)), arrival_s=t))

@@ -130,15 +130,15 @@ BATCH_T_END = 0.2
 def run_batch(precision, count, poll_every=20, seed=BATCH_SEED, max_steps=20000):
     """The batched workload: `count` seeded random Riemann problems of uneven
     size (N ~ U{64, 96, ..., 512}, transmissive boundaries, t_end = 0.2,
-    CFL 0.5, RK3) advanced together through the batch entry points until all
+    CFL 0.5, TVD-RK2, positivity-preserving limiter) advanced together through the batch entry points until all
     reach t_end; then the same problems solved one after another with the
     single-solve entry points.  Inputs are uploaded at admission (outside the
     compute but inside the wall time of both legs)."""
-    probs = cfs.riemann_batch_problems(seed, count=count, precision=precision, rk=3)
+    probs = cfs.riemann_batch_problems(seed, count=count, precision=precision, rk=2)
     total_cells = sum(c.nx * c.ny for c, _ in probs)
     arena = int(sum(cfs.Batch.footprint(c) for c, _ in probs) * 1.05) + (64 << 20)
     res = {}
-    with cfs.Batch(rk=3, precision=precision, max_problems=count, arena_bytes=arena) as b:
+    with cfs.Batch(rk=2, precision=precision, max_problems=count, arena_bytes=arena, limiter=True) as b:
         t0 = time.perf_counter()
         slot_of = {}
         for k, (cfg, U) in enumerate(probs):
@@ -194,9 +194,9 @@ def main_batch(args):
         "higher_is_better": True,
         "dtype": "f64" if prec == cfs.PREC_FP64 else "f32 (FP64 state)",
         "data": "synthetic (seeded random four-quadrant Riemann states, seed %d)" % BATCH_SEED,
-        "config": {"workload": "riemann_batch_%d_uneven_N64-512_rk3_cfl0.5_t0.2" % r["problems"],
+        "config": {"workload": "riemann_batch_%d_uneven_N64-512_rk2_cfl0.5_t0.2" % r["problems"],
                    "precision": "fp64" if prec == cfs.PREC_FP64 else "fp32_bulk",
-                   "total_cells": r["cells"], "positivity": 1, "hll_bounds": 1},
+                   "total_cells": r["cells"], "positivity_limiter": True, "hll_bounds": "velocity"},
         "batched": {"wall_s": round(r["wall"], 3), "solves_per_s": round(r["solves_per_s"], 2),
                     "admission_s": round(r["admit_s"], 3), "failed": r["failed"], "stalled": r["stalled"],
                     "steps_range": [r["steps_min"], r["steps_max"]]},
@@ -241,7 +241,7 @@ def main_serve(args):
         if burst:
             for rq in reqs:
                 rq["arrival_s"] = 0.0
-        with cfs.Batch(rk=3, precision=prec, max_problems=args.serve_slots, arena_bytes=arena) as b:
+        with cfs.Batch(rk=2, precision=prec, max_problems=args.serve_slots, arena_bytes=arena, limiter=True) as b:
             res = b.serve(reqs, steps_per_poll=args.steps_per_poll, keep_states=False)
         lat = np.array([r["latency_s"] for r in res])
         span = max(r["finished_s"] for r in res) - min(rq["arrival_s"] for rq in reqs)
@@ -264,7 +264,7 @@ def main_serve(args):
         "higher_is_better": True,
         "dtype": "f64" if prec == cfs.PREC_FP64 else "f32 (FP64 state)",
         "data": "synthetic (seeded request stream, seed %d)" % SERVE_SEED,
-        "config": {"workload": "serving_riemann128-512_cylinder256x128-1024x512_rk3_cfl0.5",
+        "config": {"workload": "serving_riemann128-1024_cylinder256x128-1024x512_mach1.5-4_t0.1-0.5_rk2_cfl0.5",
                    "arena_gb": args.serve_arena_gb, "slots": args.serve_slots,
                    "steps_per_poll": args.steps_per_poll},
         "points": points,
@@ -285,9 +285,9 @@ def main():
     ap.add_argument("--fp32-only", action="store_true", help="time the FP32-bulk leg only (profiling)")
     ap.add_argument("--workload", choices=["cylinder", "batch", "serve"], default="cylinder",
                     help="cylinder: the headline line; batch / serve: the batched-solves and serving measurements")
-    ap.add_argument("--serve-count", type=int, default=96)
-    ap.add_argument("--serve-rates", default="10,20,40,0", help="offered request rates per second (0: burst)")
-    ap.add_argument("--serve-arena-gb", type=int, default=2)
+    ap.add_argument("--serve-count", type=int, default=48)
+    ap.add_argument("--serve-rates", default="1,2,4,0", help="offered request rates per second (0: burst)")
+    ap.add_argument("--serve-arena-gb", type=int, default=4)
     ap.add_argument("--serve-slots", type=int, default=64)
     ap.add_argument("--steps-per-poll", type=int, default=5)
     ap.add_argument("--batch-count", type=int, default=256)

@@ -34,7 +34,7 @@ def test_serve_results_equal_alone(precision):
         rq["arrival_s"] = 0.0   # a burst: more than the arena holds
     fps = [cfs.Batch.footprint(r["cfg"], r["ib"]["n_ig"] if r["ib"] is not None else 0) for r in reqs]
     # room for about three problems at a time: later arrivals wait for space
-    with cfs.Batch(rk=3, precision=precision, max_problems=8, arena_bytes=3 * max(fps)) as b:
+    with cfs.Batch(rk=2, precision=precision, max_problems=8, arena_bytes=3 * max(fps), limiter=True) as b:
         res = b.serve(reqs, steps_per_poll=3)
     waited = 0
     for k, (rq, r) in enumerate(zip(reqs, res)):
