@@ -427,7 +427,8 @@ class Solver:
         _check(self._lib.cfs_set_kernel_timing(self._h, 1 if on else 0), "cfs_set_kernel_timing")
 
     def exact_tiles(self):
-        """Tiles of the last FP64 stage that took IEEE division for x/3, x/6."""
+        """Tiles of the last FP64 stage that took the IEEE-division form of WENO5
+        (a staged value outside the fast form's range); 0 in normal flows."""
         n = ctypes.c_int32(0)
         _check(self._lib.cfs_last_redo_tiles(self._h, ctypes.byref(n)), "cfs_last_redo_tiles")
         return n.value

@@ -62,7 +62,7 @@ struct StageArgs {
     double c0, c1, c2;
     int bc[4];
     double bcs[4][4];
-    int* exact_count;     // tiles of this stage outside the x/3, x/6 shortcut's exact range
+    int* exact_count;     // tiles of this stage that took the IEEE-division form of WENO5
     int* exact_other;     // the previous stage's counter: cleared by this stage
     // immersed boundary, filled while staging the tiles next to the body
     const uint8_t* tile_ib;   // per tile: its load region holds immersed-ghost cells
@@ -105,27 +105,6 @@ template <> struct K<double> {
     static constexpr double d0 = 0.1, d1 = 0.6, d2 = 0.3, half = 0.5;
     __device__ static double div(double a, double b) { return a / b; }
     __device__ static double sqrt_(double a) { return __builtin_sqrt(a); }
-    // x/3 and x/6 without the division sequence, bit-identical to IEEE x/3,
-    // x/6 for x == 0 and 2^-900 <= |x| < 2^1020: with z = RN(1/d), q0 = x*z
-    // is within 1 ulp of x/d, so r = x - d*q0 is exact (a small multiple of
-    // ulp(q0)); q0 + r*z then differs from x/d by at most |x/d|*2^-105, while
-    // x/d (d = 3*2^k) lies at least 2^(e-53)/3 from every rounding midpoint of
-    // its binade 2^e, so the single rounding of the fma gives RN(x/d).  q0 is
-    // returned for zero to keep the sign.
-    __device__ static double div3c(double x) {
-        const double z = 0x1.5555555555555p-2;
-        const double q0 = x * z;
-        const double r = __builtin_fma(-q0, 3.0, x);
-        const double qq = __builtin_fma(r, z, q0);
-        return q0 == 0.0 ? q0 : qq;
-    }
-    __device__ static double div6c(double x) {
-        const double z = 0x1.5555555555555p-3;
-        const double q0 = x * z;
-        const double r = __builtin_fma(-q0, 6.0, x);
-        const double qq = __builtin_fma(r, z, q0);
-        return q0 == 0.0 ? q0 : qq;
-    }
 };
 template <> struct K<float> {
     static constexpr float eps = 1e-2f, c1312 = 13.0f / 12.0f, q = 0.25f;
@@ -133,35 +112,88 @@ template <> struct K<float> {
     // FP32-bulk: hardware reciprocal (1 ulp) instead of the IEEE division
     __device__ static float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
     __device__ static float sqrt_(float a) { return __builtin_amdgcn_sqrtf(a); }
-    __device__ static float div3c(float x) { return x * (1.0f / 3.0f); }
-    __device__ static float div6c(float x) { return x * (1.0f / 6.0f); }
 };
 
-// Division by the constants 3 and 6 of the WENO5 candidate polynomials: the
-// shortcut above, bit-identical to IEEE division for every value that is 0 or
-// lies in [2^-900, 2^1000] (and its multiples by 5, 7, 11).  Outside that range
-// -- magnitudes below 1e-270, or inf in a state that has already blown up --
-// the result can differ from IEEE division by a few subnormal ulps (absolute
-// error below 1e-320); the stage kernel counts such tiles (exact_count).
-template <typename R> struct CDiv {
-    __device__ static R d3(R x) { return K<R>::div3c(x); }
-    __device__ static R d6(R x) { return K<R>::div6c(x); }
+// Divisions of the WENO5 reconstruction.  WOps<double, true> is the fast form
+// used on tiles whose staged values all lie in the range of fast_div_safe();
+// WOps<double, false> is plain IEEE division, used on the other tiles.  Both
+// give the IEEE quotient bit for bit on the tiles they run on.
+//
+// Division by the constants 3 and 6: q0 = x*z with z = RN(1/d), t = d*q0 - x,
+// q = q0 - t*z (two fmas).  For x == 0 or 2^-900 <= |x| <= 2^1000, q0 is within
+// 1 ulp of x/d, so t is exact (a small multiple of ulp(q0)); q0 - t*z differs
+// from x/d by at most |x/d|*2^-105, while x/d (d = 3*2^k) lies at least
+// 2^(e-53)/3 from every rounding midpoint of its binade 2^e, so the single
+// rounding of the last fma gives RN(x/d).  Signed zeros come out right without
+// a select: t is computed as d*q0 - x and negated, so an exact-zero residual is
+// -0 and q0 -+ 0 keeps the sign of q0 (= the sign of x when x is a zero).
+//
+// Division by a variable: v_rcp_f64 and two Newton steps give r ~ 1/b, then
+// q = a*r, e = a - b*q (exact), RN(q + e*r) -- the sequence the compiler emits
+// for a/b on gfx950 (v_div_scale / v_div_fmas / v_div_fixup around it), minus
+// the scaling and fix-up instructions, which leave every operand unchanged
+// when a, b > 0, a, b, 1/b and a/b are normal, |a| >= 2^-969 and
+// exponent(a) - exponent(b) < 768.  In WENO5 the numerators are 0.1, 0.6,
+// 0.3 and the alpha_k, the denominators (eps + beta)^2 >= 1e-4 and
+// sum(alpha) <= 1e4; with every stencil value at most 2^200 in magnitude,
+// beta <= 2^406, so alpha_k >= 2^-816 and every condition holds.  The three
+// weights alpha_k / sum share one reciprocal (the same r the compiler's
+// sequence computes for each of them).
+template <typename R, bool FAST> struct WOps;
+template <> struct WOps<double, true> {
+    __device__ static double d3(double x) {
+        const double z = 0x1.5555555555555p-2;
+        const double q0 = x * z;
+        const double t = __builtin_fma(q0, 3.0, -x);
+        return __builtin_fma(-t, z, q0);
+    }
+    __device__ static double d6(double x) {
+        const double z = 0x1.5555555555555p-3;
+        const double q0 = x * z;
+        const double t = __builtin_fma(q0, 6.0, -x);
+        return __builtin_fma(-t, z, q0);
+    }
+    __device__ static double rcp(double b) {
+        double r = __builtin_amdgcn_rcp(b);
+        r = __builtin_fma(r, __builtin_fma(-b, r, 1.0), r);
+        return __builtin_fma(r, __builtin_fma(-b, r, 1.0), r);
+    }
+    __device__ static double div_r(double a, double b, double r) {
+        const double q = a * r;
+        return __builtin_fma(__builtin_fma(-b, q, a), r, q);
+    }
+    __device__ static double div(double a, double b) { return div_r(a, b, rcp(b)); }
+};
+template <> struct WOps<double, false> {
+    __device__ static double d3(double x) { return x / 3.0; }
+    __device__ static double d6(double x) { return x / 6.0; }
+    __device__ static double rcp(double b) { return b; }
+    __device__ static double div_r(double a, double b, double) { return a / b; }
+    __device__ static double div(double a, double b) { return a / b; }
+};
+template <bool FAST> struct WOps<float, FAST> {
+    __device__ static float d3(float x) { return x * (1.0f / 3.0f); }
+    __device__ static float d6(float x) { return x * (1.0f / 6.0f); }
+    __device__ static float rcp(float b) { return __builtin_amdgcn_rcpf(b); }
+    __device__ static float div_r(float a, float, float r) { return a * r; }
+    __device__ static float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 };
 
-// range in which div3c/div6c are exact for every staged value and its
-// multiples by 5, 7 and 11
+// range of the staged values for which WOps<double, true> is exact: every
+// value, and its multiples by 5, 7 and 11, inside the constant-division range,
+// and the smoothness indicators small enough for the unscaled division
 __device__ __forceinline__ bool fast_div_safe(double v) {
     const double a = __builtin_fabs(v);
-    return v == 0.0 || (a >= 0x1p-900 && a <= 0x1p+1000);
+    return v == 0.0 || (a >= 0x1p-900 && a <= 0x1p+200);
 }
 
 // WENO5 value at the face from five cell values, v2 the upwind cell
 // (source/interpolation/Weno5th.cpp:4-28: Jiang-Shu smoothness indicators,
 // eps = 1e-2, ideal weights 0.1/0.6/0.3).  Operation order as the reference.
-template <typename R>
+template <typename R, bool FAST>
 __device__ __forceinline__ R weno5(R v0, R v1, R v2, R v3, R v4) {
     using C = K<R>;
-    using D = CDiv<R>;
+    using D = WOps<R, FAST>;
     const R s0 = (v0 - R(2) * v1) + v2;
     const R t0 = (v0 - R(4) * v1) + R(3) * v2;
     const R b0 = C::c1312 * (s0 * s0) + C::q * (t0 * t0);
@@ -172,11 +204,12 @@ __device__ __forceinline__ R weno5(R v0, R v1, R v2, R v3, R v4) {
     const R t2 = (R(3) * v2 - R(4) * v3) + v4;
     const R b2 = C::c1312 * (s2 * s2) + C::q * (t2 * t2);
     const R e0 = C::eps + b0, e1 = C::eps + b1, e2 = C::eps + b2;
-    const R a0 = C::div(C::d0, e0 * e0);
-    const R a1 = C::div(C::d1, e1 * e1);
-    const R a2 = C::div(C::d2, e2 * e2);
+    const R a0 = D::div(C::d0, e0 * e0);
+    const R a1 = D::div(C::d1, e1 * e1);
+    const R a2 = D::div(C::d2, e2 * e2);
     const R sum = (a0 + a1) + a2;
-    const R w0 = C::div(a0, sum), w1 = C::div(a1, sum), w2 = C::div(a2, sum);
+    const R rs = D::rcp(sum);
+    const R w0 = D::div_r(a0, sum, rs), w1 = D::div_r(a1, sum, rs), w2 = D::div_r(a2, sum, rs);
     const R q0 = (D::d3(v0) - D::d6(R(7) * v1)) + D::d6(R(11) * v2);
     const R q1 = (D::d6(-v1) + D::d6(R(5) * v2)) + D::d3(v3);
     const R q2 = (D::d3(v2) + D::d6(R(5) * v3)) - D::d6(v4);
@@ -191,7 +224,7 @@ __device__ __forceinline__ R weno5(R v0, R v1, R v2, R v3, R v4) {
 // The x and y forms of the reference map onto this one form because IEEE
 // addition and multiplication are commutative (mx*my == my*mx bit for bit).
 // out = (mass, normal momentum, tangential momentum, energy) flux.
-template <typename R>
+template <typename R, bool FAST>
 __device__ __forceinline__ void face_flux(const R* __restrict__ S, const int cell[6], int nrm, int tng,
                                           R g, int positivity, int hll_bounds, R out[4]) {
     using C = K<R>;
@@ -202,8 +235,8 @@ __device__ __forceinline__ void face_flux(const R* __restrict__ S, const int cel
         R v[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) v[k] = S[cell[k] + comp[q]];
-        L[q] = weno5<R>(v[0], v[1], v[2], v[3], v[4]);
-        Rr[q] = weno5<R>(v[5], v[4], v[3], v[2], v[1]);
+        L[q] = weno5<R, FAST>(v[0], v[1], v[2], v[3], v[4]);
+        Rr[q] = weno5<R, FAST>(v[5], v[4], v[3], v[2], v[1]);
         cl[q] = v[2];
         cr[q] = v[3];
     }
@@ -403,8 +436,9 @@ __device__ __forceinline__ void block_max_to(double m, unsigned long long* acc, 
 
 // ---------------------------------------------------------------------------
 // One tile: stage it (with halo) in LDS, compute its 511 faces, update its
-// cells.  Tiles holding a value outside the range where the WENO5 constant
-// divisions are bit-identical to IEEE division are counted in exact_count.
+// cells.  Tiles holding a value outside the range of the fast WENO5 division
+// forms (fast_div_safe) compute their faces with IEEE division and are
+// counted in exact_count.
 // ---------------------------------------------------------------------------
 // Positivity-preserving flux limiter (positivity = 2; no reference
 // counterpart).  Hu, Adams & Shu, J. Comput. Phys. 242 (2013) 169-180: each
@@ -589,6 +623,71 @@ __device__ __forceinline__ void viscous_flux(const ViscLDS<R>& V, const int cs[6
         out[q] = ((R(-1.0) / R(24.0)) * acc[0][q] + (R(26.0) / R(24.0)) * acc[1][q]) + (R(-1.0) / R(24.0)) * acc[2][q];
 }
 
+// Phase 1 of a tile: its 511 faces, fluxes to LDS.  FAST selects the WENO5
+// division forms (WOps); the caller picks it per tile.
+template <typename R, bool VISC, bool LIMIT, bool FAST>
+__device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, double dt, const R* S, R* F,
+                                           const ViscLDS<R>* V, const LimitLDS<R>* Lm) {
+    const int tid = threadIdx.x;
+    const bool px = a.bc[0] == BC_PERIODIC, py = a.bc[2] == BC_PERIODIC;
+    const R g = (R)a.gamma;
+    // Lax-Friedrichs shares of the limiter: dt/dx and dt/dy
+    const R lam_x = (R)(dt / a.dx), lam_y = (R)(dt / a.dy);
+    for (int s = tid; s < NXF + NYF; s += BLOCK) {
+        int cell[6], nrm, tng;
+        if (s < NXF) {
+            const int fr = s / TJ, jl = s - fr * TJ;
+            const int f = i0 + fr;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                int o = k - 3;
+                // boundary faces: the walk stops at the boundary face, so the
+                // stencil is [G,G,G,c0,c0,c0] at f = 0 and [c,c,c,G,G,G] at f = n
+                if (!px && f == 0 && o > 0) o = 0;
+                if (!px && f == a.nx && o < -1) o = -1;
+                cell[k] = ((fr + o + HALO) * SC + jl + HALO) * 4;
+            }
+            nrm = 1; tng = 2;
+        } else {
+            const int t = s - NXF;
+            const int il = t / (TJ + 1), gl = t - il * (TJ + 1);
+            const int gf = j0 + gl;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                int o = k - 3;
+                if (!py && gf == 0 && o > 0) o = 0;
+                if (!py && gf == a.ny && o < -1) o = -1;
+                cell[k] = ((il + HALO) * SC + gl + o + HALO) * 4;
+            }
+            nrm = 2; tng = 1;
+        }
+        R fl[4];
+        face_flux<R, FAST>(S, cell, nrm, tng, g, a.positivity, a.hll_bounds, fl);
+        if constexpr (LIMIT) {
+            if (a.positivity == 2) {
+                const bool xf = s < NXF;
+                limit_face<R>(S, *Lm, cell[2] >> 2, cell[3] >> 2, nrm, tng, xf ? SC : 1, xf ? 1 : SC,
+                              xf ? lam_x : lam_y, xf ? lam_y : lam_x, g, fl);
+            }
+        }
+        if constexpr (VISC) {   // net flux = inviscid - viscous (mass: minus zero)
+            int cs[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) cs[k] = cell[k] >> 2;
+            const bool xface = s < NXF;
+            R fv[3];
+            viscous_flux<R>(*V, cs, xface, xface ? (R)a.dx : (R)a.dy, (R)a.mu, (R)a.kappa, fv);
+            fl[1] = fl[1] - fv[0];
+            fl[2] = fl[2] - fv[1];
+            fl[3] = fl[3] - fv[2];
+        }
+        F[s * 4 + 0] = fl[0];
+        F[s * 4 + nrm] = fl[1];
+        F[s * 4 + tng] = fl[2];
+        F[s * 4 + 3] = fl[3];
+    }
+}
+
 template <typename R, bool VISC, bool LIMIT>
 __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R* F, double* wmax,
                                            ViscLDS<R>* V, LimitLDS<R>* Lm) {
@@ -665,9 +764,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
             if (sizeof(R) == 8) unsafe |= !fast_div_safe(v[q]);
         }
     }
-    // one barrier + OR over the tile: count tiles outside the range where the
-    // constant-division shortcut is bit-identical to IEEE division
-    if (__syncthreads_or(unsafe) && tid == 0) atomicAdd(a.exact_count, 1);
+    // one barrier + OR over the tile: tiles outside the range of the fast
+    // WENO5 division forms take the IEEE form (and are counted)
+    const bool tile_unsafe = __syncthreads_or(unsafe) != 0;
+    if (tile_unsafe && tid == 0) atomicAdd(a.exact_count, 1);
 
     if constexpr (LIMIT) {
         if (a.positivity == 2) {
@@ -725,64 +825,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
         __syncthreads();
     }
 
-    // ---- faces ----
-    const bool px = a.bc[0] == BC_PERIODIC, py = a.bc[2] == BC_PERIODIC;
-    const R g = (R)a.gamma;
-    // Lax-Friedrichs shares of the limiter: dt/dx and dt/dy
-    const R lam_x = (R)(dt / a.dx), lam_y = (R)(dt / a.dy);
-    for (int s = tid; s < NXF + NYF; s += BLOCK) {
-        int cell[6], nrm, tng;
-        if (s < NXF) {
-            const int fr = s / TJ, jl = s - fr * TJ;
-            const int f = i0 + fr;
-#pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                int o = k - 3;
-                // boundary faces: the walk stops at the boundary face, so the
-                // stencil is [G,G,G,c0,c0,c0] at f = 0 and [c,c,c,G,G,G] at f = n
-                if (!px && f == 0 && o > 0) o = 0;
-                if (!px && f == a.nx && o < -1) o = -1;
-                cell[k] = ((fr + o + HALO) * SC + jl + HALO) * 4;
-            }
-            nrm = 1; tng = 2;
-        } else {
-            const int t = s - NXF;
-            const int il = t / (TJ + 1), gl = t - il * (TJ + 1);
-            const int gf = j0 + gl;
-#pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                int o = k - 3;
-                if (!py && gf == 0 && o > 0) o = 0;
-                if (!py && gf == a.ny && o < -1) o = -1;
-                cell[k] = ((il + HALO) * SC + gl + o + HALO) * 4;
-            }
-            nrm = 2; tng = 1;
-        }
-        R fl[4];
-        face_flux<R>(S, cell, nrm, tng, g, a.positivity, a.hll_bounds, fl);
-        if constexpr (LIMIT) {
-            if (a.positivity == 2) {
-                const bool xf = s < NXF;
-                limit_face<R>(S, *Lm, cell[2] >> 2, cell[3] >> 2, nrm, tng, xf ? SC : 1, xf ? 1 : SC,
-                              xf ? lam_x : lam_y, xf ? lam_y : lam_x, g, fl);
-            }
-        }
-        if constexpr (VISC) {   // net flux = inviscid - viscous (mass: minus zero)
-            int cs[6];
-#pragma unroll
-            for (int k = 0; k < 6; ++k) cs[k] = cell[k] >> 2;
-            const bool xface = s < NXF;
-            R fv[3];
-            viscous_flux<R>(*V, cs, xface, xface ? (R)a.dx : (R)a.dy, (R)a.mu, (R)a.kappa, fv);
-            fl[1] = fl[1] - fv[0];
-            fl[2] = fl[2] - fv[1];
-            fl[3] = fl[3] - fv[2];
-        }
-        F[s * 4 + 0] = fl[0];
-        F[s * 4 + nrm] = fl[1];
-        F[s * 4 + tng] = fl[2];
-        F[s * 4 + 3] = fl[3];
-    }
+    // ---- faces: the exact-division form on tiles outside the fast range ----
+    if (sizeof(R) == 8 && tile_unsafe)
+        tile_faces<R, VISC, LIMIT, false>(a, i0, j0, dt, S, F, V, Lm);
+    else
+        tile_faces<R, VISC, LIMIT, true>(a, i0, j0, dt, S, F, V, Lm);
     __syncthreads();
 
     // ---- cell update: flux divergence (Cell::comRHS, Cell.cpp:110-125) and the

@@ -194,11 +194,12 @@ int32_t cfs_checksums(cfs_solver* s, double out[4]);
  * No reference counterpart: it calibrates hardware byte counters against a
  * kernel of known traffic. */
 int32_t cfs_calibrate_copy(cfs_solver* s, int32_t repeats);
-/* Number of 15x16-cell tiles of the last FP64 stage holding a value outside
- * the range (0, or |v| in [2^-900, 2^1000]) in which the kernel's division-free
- * evaluation of the WENO5 divisions by 3 and 6 is bit-identical to IEEE
- * division.  Outside it the two differ by at most a few subnormal ulps
- * (absolute error below 1e-320); 0 means the stage was bit-faithful. */
+/* Number of 15x16-cell tiles of the last FP64 stage that computed their
+ * faces with plain IEEE division in WENO5 because they hold a value outside
+ * the range (0, or |v| in [2^-900, 2^200]) of the kernel's faster division
+ * forms (x/3 and x/6 by two fmas, the weights through one refined
+ * reciprocal), which are bit-identical to IEEE division inside it.  Either
+ * way the stage is bit-identical to IEEE evaluation; 0 in normal flows. */
 int32_t cfs_last_redo_tiles(cfs_solver* s, int32_t* n);
 /* The right-hand side L(U) = -(F_{i+1/2} - F_{i-1/2})/dx - (G_{j+1/2} -
  * G_{j-1/2})/dy that the last stage applied, [4][nx][ny], on fluid cells (0

@@ -1,9 +1,15 @@
-/* Checks that division of a double by 3 or 6 computed as
- *   q0 = x*z;  r = fma(-q0, d, x);  q = fma(r, z, q0)   (z = RN(1/d))
- * with q0 returned when it is zero (keeps the sign of zero)
- * equals the IEEE quotient x/d bit for bit.  Random doubles over the normal
- * range, integers, and x = d*y +- a few ulp (quotients next to rounding
- * boundaries).  Prints the number of mismatches; argument: samples. */
+/* Checks two division forms of the FP64 stage kernel against IEEE division,
+ * bit for bit.
+ * 1. Division of a double by 3 or 6 as
+ *      q0 = x*z;  t = fma(q0, d, -x);  q = fma(-t, z, q0)   (z = RN(1/d)),
+ *    signed zeros included: random doubles over the normal range, integers,
+ *    and x = d*y +- a few ulp (quotients next to rounding boundaries).
+ * 2. Division a/b by a reciprocal refined with two Newton steps,
+ *      r = fma(r, fma(-b, r, 1), r) (twice);  q = a*r;  q = fma(fma(-b, q, a), r, q),
+ *    from a starting reciprocal with a relative error up to 2^-22 (the
+ *    accuracy of the hardware reciprocal), for a, b > 0 in the ranges the
+ *    WENO5 weights take: b in [1e-4, 2^812], a/b normal, a >= 2^-969.
+ * Prints the number of mismatches; argument: samples. */
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -16,9 +22,26 @@ static double bits(uint64_t b) { double d; memcpy(&d, &b, 8); return d; }
 
 static double fastdiv(double x, double d, double z) {
     double q0 = x * z;
-    double r = fma(-q0, d, x);
-    double q = fma(r, z, q0);
-    return q0 == 0.0 ? q0 : q;
+    double t = fma(q0, d, -x);
+    return fma(-t, z, q0);
+}
+
+/* starting reciprocal: 1/b with a random relative error in [-2^-22, 2^-22] */
+static double rough_rcp(double b, uint64_t u) {
+    double e = ((double)(int64_t)(u >> 11) / 4503599627370496.0 - 1.0) * 0x1p-22;
+    return (1.0 / b) * (1.0 + e);
+}
+
+static double nrdiv(double a, double b, double r) {
+    r = fma(r, fma(-b, r, 1.0), r);
+    r = fma(r, fma(-b, r, 1.0), r);
+    double q = a * r;
+    return fma(fma(-b, q, a), r, q);
+}
+
+static double rand_in(uint64_t b, int e_lo, int e_hi) {   /* positive, exponent in [e_lo, e_hi] */
+    uint64_t e = (uint64_t)(1023 + e_lo + (int)((b >> 12) % (uint64_t)(e_hi - e_lo + 1)));
+    return bits((b & 0x000FFFFFFFFFFFFFull) | (e << 52));
 }
 
 int main(int argc, char** argv) {
@@ -57,6 +80,19 @@ int main(int argc, char** argv) {
         for (int t = 0; t < 2; ++t) {
             double a = zz[t] / d, f = fastdiv(zz[t], d, z);
             if (memcmp(&a, &f, 8) != 0) ++bad;
+        }
+    }
+    /* variable divisor, refined reciprocal */
+    for (long i = 0; i < n; ++i) {
+        uint64_t u1 = next(), u2 = next(), u3 = next();
+        double b = (i & 1) ? rand_in(u1, -14, 812) : rand_in(u1, -14, 20);
+        double a = (i & 2) ? rand_in(u2, -969, 14) : rand_in(u2, -10, 14);
+        double ex = a / b;
+        if (!(ex >= 0x1p-1022) || !isfinite(ex)) continue;
+        double f = nrdiv(a, b, rough_rcp(b, u3));
+        if (memcmp(&ex, &f, 8) != 0) {
+            if (bad < 10) printf("mismatch a=%a b=%a: %a vs %a\n", a, b, ex, f);
+            ++bad;
         }
     }
     printf("%ld\n", bad);

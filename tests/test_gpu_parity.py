@@ -216,14 +216,20 @@ def test_state_sequence_errors():
         s.set_state(cfs.riemann_quadrants(cfg))
 
 
-def test_constant_division_range_counter():
-    # momentum of order 1e-300 in a patch: outside the range where the kernel's
-    # division-free x/3, x/6 is bit-identical to IEEE division.  The tiles are
-    # counted, and the states stay within the parity tolerance.
-    cfg = cfs.make_config(64, 48, 1.0, 0.75, rk=3)
+@pytest.mark.parametrize("patch", ["tiny", "huge"])
+def test_exact_division_tiles(patch):
+    # values outside the range where the kernel's fast WENO5 divisions are
+    # bit-identical to IEEE division (momentum of order 1e-300 in patches, or
+    # the whole state scaled by 1e65 > 2^200, a flow the Euler equations map
+    # onto the unscaled one): those tiles take the IEEE-division path, are
+    # counted, and the states stay bit-identical to the oracle
+    cfg = cfs.make_config(64, 48, 1.0, 0.75, rk=3, hll_bounds=1 if patch == "huge" else 0)
     U = cfs.riemann_quadrants(cfg)
-    U[2, 20:30, 10:20] = 1e-300
-    U[1, 40:44, 30:34] = -3e-305
+    if patch == "tiny":
+        U[2, 20:30, 10:20] = 1e-300
+        U[1, 40:44, 30:34] = -3e-305
+    else:
+        U *= 1e65
     s, o = _pair(cfg, U)
     dt = 0.4 / o.cfl_smax()
     counted = []
@@ -231,15 +237,14 @@ def test_constant_division_range_counter():
         s.stage(i, dt)
         o.stage(i, dt)
         counted.append(s.exact_tiles())
-        g, r = s.get_state(), o.get_state()
-        scale = np.maximum(1.0, np.abs(r).reshape(4, -1).max(axis=1))
-        assert np.all(np.abs(g - r).reshape(4, -1).max(axis=1) <= TOL * scale)
-    print("tiles outside the bit-identical range per stage:", counted)
+        _assert_parity(s.get_state(), o.get_state(), "%s patch, stage %d" % (patch, i))
+    print("tiles on the IEEE-division path per stage:", counted)
     assert counted[0] > 0
-    # a normal-range state: no tile counted, bit-identical
+    # a normal-range state: no tile counted
     cfg = cfs.make_config(64, 48, 1.0, 0.75, rk=3)
     U = cfs.riemann_quadrants(cfg)
     s, o = _pair(cfg, U)
+    dt = 0.4 / o.cfl_smax()
     for i in range(1, 7):
         s.stage(i, dt)
         o.stage(i, dt)

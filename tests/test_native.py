@@ -1,9 +1,11 @@
 """Native (C) checks of arithmetic shortcuts used by the kernels.
 
 const_div_check.c: the division-free x/3, x/6 of the FP64 stage kernel
-(q0 = x*z; r = fma(-q0, d, x); q = fma(r, z, q0)) equals IEEE x/d bit for bit on
+(q0 = x*z; t = fma(q0, d, -x); q = fma(-t, z, q0)) equals IEEE x/d bit for bit on
 random normal-range doubles, integers, quotients next to rounding boundaries
-and signed zeros.
+and signed zeros; and the WENO5 weight division through a reciprocal refined
+by two Newton steps (started with a relative error up to 2^-22) equals IEEE a/b
+over the operand ranges the weights take.
 """
 import os
 import shutil
