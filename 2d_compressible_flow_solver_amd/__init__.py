@@ -210,7 +210,7 @@ class BatchConfig(ctypes.Structure):
     """cfs_batch_config of include/cfs.h."""
     _fields_ = [("rk", ctypes.c_int32), ("precision", ctypes.c_int32), ("max_problems", ctypes.c_int32),
                 ("workgroups", ctypes.c_int32), ("arena_bytes", ctypes.c_int64), ("limiter", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("tile_shape", ctypes.c_int32)]
 
 
 class Immersed(ctypes.Structure):
@@ -282,6 +282,14 @@ def _lib():
     lib.cfs_batch_free_bytes.argtypes = [vp, P(ctypes.c_int64)]
     lib.cfs_batch_capacity.argtypes = [vp, P(ctypes.c_int64), P(i32)]
     lib.cfs_serve.argtypes = [vp, P(Request), i32, i32]
+    lib.cfs_tile_shape_info.argtypes = [i32, P(i32), P(i32), P(i32)]
+    lib.cfs_set_tile_shape.argtypes = [vp, i32]
+    lib.cfs_get_tile_shape.argtypes = [vp, P(i32)]
+    lib.cfs_autotune.argtypes = [vp, i32, P(i32), vp]
+    lib.cfs_batch_autotune.argtypes = [P(BatchConfig), P(Request), i32, i32, P(i32), vp]
+    for name in ("cfs_tile_shape_info", "cfs_set_tile_shape", "cfs_get_tile_shape", "cfs_autotune",
+                 "cfs_batch_autotune"):
+        getattr(lib, name).restype = i32
     for name in ("cfs_batch_create", "cfs_batch_admit", "cfs_batch_step", "cfs_batch_poll", "cfs_batch_retire",
                  "cfs_batch_free_bytes", "cfs_batch_capacity", "cfs_serve"):
         getattr(lib, name).restype = i32
@@ -309,6 +317,17 @@ def _ptr(a):
 
 def device_count():
     return _lib().cfs_device_count()
+
+
+NUM_TILE_SHAPES = 3
+
+
+def tile_shape_info(shape):
+    """(ti, tj, threads) of stage-kernel tile shape `shape`."""
+    ti, tj, nt = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    _check(_lib().cfs_tile_shape_info(shape, ctypes.byref(ti), ctypes.byref(tj), ctypes.byref(nt)),
+           "cfs_tile_shape_info")
+    return ti.value, tj.value, nt.value
 
 
 def circle_setup(cfg, cx, cy, radius, n_lagrangian=400, n_layers=3, interp=IB_BILINEAR_POSITIVE):
@@ -444,6 +463,23 @@ class Solver:
         _check(self._lib.cfs_last_kernel_ms(self._h, ctypes.byref(ms)), "cfs_last_kernel_ms")
         return ms.value
 
+    @property
+    def tile_shape(self):
+        k = ctypes.c_int32(0)
+        _check(self._lib.cfs_get_tile_shape(self._h, ctypes.byref(k)), "cfs_get_tile_shape")
+        return k.value
+
+    def set_tile_shape(self, shape):
+        _check(self._lib.cfs_set_tile_shape(self._h, shape), "cfs_set_tile_shape")
+
+    def autotune(self, reps=20):
+        """Times every tile shape on the current state, selects the fastest.
+        Returns (best shape, [ms per stage-kernel launch of each shape])."""
+        best = ctypes.c_int32(0)
+        ms = np.zeros(NUM_TILE_SHAPES)
+        _check(self._lib.cfs_autotune(self._h, reps, ctypes.byref(best), _ptr(ms)), "cfs_autotune")
+        return best.value, [float(x) for x in ms]
+
 
 class Batch:
     """Many independent solves advanced together (a cfs_batch handle).
@@ -454,9 +490,10 @@ class Batch:
     frees its slot."""
 
     def __init__(self, rk=3, precision=PREC_FP64, max_problems=256, arena_bytes=1 << 30, workgroups=0,
-                 stream=None, limiter=False):
+                 stream=None, limiter=False, tile_shape=0):
         self._lib = _lib()
-        self.bc = BatchConfig(rk, precision, max_problems, workgroups, arena_bytes, 1 if limiter else 0, 0)
+        self.bc = BatchConfig(rk, precision, max_problems, workgroups, arena_bytes, 1 if limiter else 0,
+                              tile_shape)
         h = ctypes.c_void_p()
         _check(self._lib.cfs_batch_create(ctypes.byref(self.bc), stream, ctypes.byref(h)), "cfs_batch_create")
         self._h = h
@@ -533,31 +570,7 @@ class Batch:
         arrival_s)].  Returns one dict per request with admitted_s,
         finished_s, latency_s, t, steps, state and (keep_states) U."""
         n = len(requests)
-        arr = (Request * n)()
-        keep = []
-        outs = []
-        for k, rq in enumerate(requests):
-            cfg = rq["cfg"]
-            U = np.ascontiguousarray(rq["U"], dtype=np.float64)
-            keep += [cfg, U]
-            r = arr[k]
-            r.cfg = ctypes.cast(ctypes.pointer(cfg), ctypes.c_void_p)
-            r.U = _ptr(U).value
-            if rq.get("ib") is not None:
-                ib = rq["ib"]
-                arrs = [np.ascontiguousarray(ib["cell_type"], dtype=np.uint8),
-                        np.ascontiguousarray(ib["ig_cells"], dtype=np.int32),
-                        np.ascontiguousarray(ib["donors"], dtype=np.int32),
-                        np.ascontiguousarray(ib["image_xy"], dtype=np.float64),
-                        np.ascontiguousarray(ib["coef"], dtype=np.float64),
-                        np.ascontiguousarray(ib["transp"], dtype=np.int32)]
-                imm = Immersed(ib["interp"], int(ib["n_ig"]), *[_ptr(a) for a in arrs])
-                keep += arrs + [imm]
-                r.ib = ctypes.cast(ctypes.pointer(imm), ctypes.c_void_p)
-            r.cfl, r.t_end, r.arrival_s = rq.get("cfl", 0.5), rq["t_end"], rq.get("arrival_s", 0.0)
-            out = np.empty((4, cfg.nx, cfg.ny)) if keep_states else None
-            outs.append(out)
-            r.U_out = _ptr(out).value if keep_states else None
+        arr, keep, outs = _request_array(requests, keep_states)
         _check(self._lib.cfs_serve(self._h, arr, n, steps_per_poll), "cfs_serve")
         res = []
         for k in range(n):
@@ -567,6 +580,56 @@ class Batch:
                             state=r.state, U=outs[k]))
         del keep
         return res
+
+
+def _request_array(requests, keep_states=False):
+    """cfs_request array of request dicts (cfg, U, ib=None, cfl, t_end,
+    arrival_s); returns (array, objects to keep alive, output states)."""
+    n = len(requests)
+    arr = (Request * n)()
+    keep = []
+    outs = []
+    for k, rq in enumerate(requests):
+        cfg = rq["cfg"]
+        U = np.ascontiguousarray(rq["U"], dtype=np.float64)
+        keep += [cfg, U]
+        r = arr[k]
+        r.cfg = ctypes.cast(ctypes.pointer(cfg), ctypes.c_void_p)
+        r.U = _ptr(U).value
+        if rq.get("ib") is not None:
+            ib = rq["ib"]
+            arrs = [np.ascontiguousarray(ib["cell_type"], dtype=np.uint8),
+                    np.ascontiguousarray(ib["ig_cells"], dtype=np.int32),
+                    np.ascontiguousarray(ib["donors"], dtype=np.int32),
+                    np.ascontiguousarray(ib["image_xy"], dtype=np.float64),
+                    np.ascontiguousarray(ib["coef"], dtype=np.float64),
+                    np.ascontiguousarray(ib["transp"], dtype=np.int32)]
+            imm = Immersed(ib["interp"], int(ib["n_ig"]), *[_ptr(a) for a in arrs])
+            keep += arrs + [imm]
+            r.ib = ctypes.cast(ctypes.pointer(imm), ctypes.c_void_p)
+        r.cfl, r.t_end, r.arrival_s = rq.get("cfl", 0.5), rq["t_end"], rq.get("arrival_s", 0.0)
+        out = np.empty((4, cfg.nx, cfg.ny)) if keep_states else None
+        outs.append(out)
+        r.U_out = _ptr(out).value if keep_states else None
+    return arr, keep, outs
+
+
+def batch_autotune(problems, rk=3, precision=PREC_FP64, limiter=False, steps=10, arena_bytes=1 << 30,
+                   max_problems=256, workgroups=0):
+    """Tile-shape autotuner of the batched kernel (cfs_batch_autotune): times
+    `steps` RK steps of a batch of `problems` (request dicts with cfg, U,
+    optional ib, cfl; or (cfg, U) pairs) with every tile shape.  Returns
+    (best shape, [ms per step of each shape])."""
+    reqs = [p if isinstance(p, dict) else dict(cfg=p[0], U=p[1], ib=None, cfl=0.5, t_end=0.0) for p in problems]
+    reqs = [dict(r, t_end=r.get("t_end", 0.0)) for r in reqs]
+    arr, keep, _ = _request_array(reqs, keep_states=False)
+    bc = BatchConfig(rk, precision, max(max_problems, len(reqs)), workgroups, arena_bytes, 1 if limiter else 0, 0)
+    best = ctypes.c_int32(0)
+    ms = np.zeros(NUM_TILE_SHAPES)
+    _check(_lib().cfs_batch_autotune(ctypes.byref(bc), arr, len(reqs), steps, ctypes.byref(best), _ptr(ms)),
+           "cfs_batch_autotune")
+    del keep
+    return best.value, [float(x) for x in ms]
 
 
 # ---------------------------------------------------------------------------

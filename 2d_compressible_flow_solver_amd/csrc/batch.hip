@@ -77,7 +77,7 @@ struct Layout {
 
 Layout layout_of(const cfs_config& c, int n_ig, bool has_ib) {
     const size_t plane = (size_t)c.nx * c.ny;
-    const size_t tiles = (size_t)((c.nx + TI - 1) / TI) * ((c.ny + TJ - 1) / TJ);
+    const size_t tiles = (size_t)tiles_for(c, 0);   // the smallest tile shape: the most tiles
     Layout L;
     size_t o = 0;
     L.planes = o;
@@ -141,12 +141,31 @@ struct cfs_batch {
 
 namespace {
 
-template <typename R, bool LIMIT>
-int launch_batch_stage(cfs_batch* b, int parity, int sub) {
-    hipLaunchKernelGGL((batch_stage_kernel<R, LIMIT>), dim3((unsigned)b->grid), dim3(BLOCK), 0, b->stream, b->descs,
-                       b->tiles, b->ntiles, parity, sub);
+template <typename R, bool LIMIT, typename Sh>
+int launch_batch_shape(cfs_batch* b, int parity, int sub) {
+    hipLaunchKernelGGL((batch_stage_kernel<R, LIMIT, Sh>), dim3((unsigned)b->grid), dim3(Sh::NT), 0, b->stream,
+                       b->descs, b->tiles, b->ntiles, parity, sub);
     HIP_OK(hipGetLastError());
     return CFS_OK;
+}
+
+template <typename R, bool LIMIT>
+int launch_batch_stage(cfs_batch* b, int parity, int sub) {
+    const int k = b->bc.tile_shape;
+    return k == 1 ? launch_batch_shape<R, LIMIT, Shape1>(b, parity, sub)
+                  : k == 2 ? launch_batch_shape<R, LIMIT, Shape2>(b, parity, sub)
+                           : launch_batch_shape<R, LIMIT, Shape0>(b, parity, sub);
+}
+
+template <typename R, bool LIMIT, typename Sh>
+hipError_t blocks_per_cu(int* per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, batch_stage_kernel<R, LIMIT, Sh>, Sh::NT, 0);
+}
+
+template <typename R, bool LIMIT>
+hipError_t blocks_per_cu_shape(int k, int* per_cu) {
+    return k == 1 ? blocks_per_cu<R, LIMIT, Shape1>(per_cu)
+                  : k == 2 ? blocks_per_cu<R, LIMIT, Shape2>(per_cu) : blocks_per_cu<R, LIMIT, Shape0>(per_cu);
 }
 
 int launch_batch_step(cfs_batch* b, int parity) {
@@ -186,7 +205,7 @@ void drop_graph(cfs_batch* b) {
     b->graph = nullptr;
 }
 
-long long tiles_of(const cfs_config& c) { return (long long)((c.nx + TI - 1) / TI) * ((c.ny + TJ - 1) / TJ); }
+long long tiles_of(const cfs_batch* b, const cfs_config& c) { return tiles_for(c, b->bc.tile_shape); }
 
 }  // namespace
 
@@ -205,6 +224,7 @@ int32_t cfs_batch_create(const cfs_batch_config* bc, void* hip_stream, cfs_batch
     if (bc->max_problems < 1 || bc->max_problems > (1 << 20) || bc->arena_bytes < 1 || bc->workgroups < 0)
         return CFS_E_ARG;
     if (bc->limiter != 0 && bc->limiter != 1) return CFS_E_ARG;
+    if (bc->tile_shape < 0 || bc->tile_shape >= NUM_SHAPES) return CFS_E_ARG;
     cfs_batch* b = new (std::nothrow) cfs_batch();
     if (!b) return CFS_E_OOM;
     b->bc = *bc;
@@ -221,18 +241,18 @@ int32_t cfs_batch_create(const cfs_batch_config* bc, void* hip_stream, cfs_batch
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e == hipSuccess) {
         const bool fp64 = bc->precision == CFS_PREC_FP64, lim = bc->limiter != 0;
-        e = fp64 ? (lim ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_stage_kernel<double, true>, BLOCK, 0)
-                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_stage_kernel<double, false>, BLOCK, 0))
-                 : (lim ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_stage_kernel<float, true>, BLOCK, 0)
-                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_stage_kernel<float, false>, BLOCK, 0));
+        const int k = bc->tile_shape;
+        e = fp64 ? (lim ? blocks_per_cu_shape<double, true>(k, &per_cu) : blocks_per_cu_shape<double, false>(k, &per_cu))
+                 : (lim ? blocks_per_cu_shape<float, true>(k, &per_cu) : blocks_per_cu_shape<float, false>(k, &per_cu));
     }
     if (e != hipSuccess) { cfs_batch_destroy(b); return CFS_E_HIP; }
     int g = bc->workgroups > 0 ? bc->workgroups : cus * (per_cu > 0 ? per_cu : 1);
     b->grid = ((g + 7) / 8) * 8;
     // tile list capacity: twice the full tiles an arena of problem planes
-    // holds (96 B per cell, 240 cells per tile) plus 16 ragged tiles per
-    // slot; admission past it is CFS_E_FULL
-    b->tiles_cap = bc->arena_bytes / (96 * TI * TJ / 2) + 16ll * bc->max_problems;
+    // holds (96 B per cell) plus 16 ragged tiles per slot; admission past it
+    // is CFS_E_FULL
+    const ShapeDims sd = shape_dims(bc->tile_shape);
+    b->tiles_cap = bc->arena_bytes / (96ll * sd.ti * sd.tj / 2) + 16ll * bc->max_problems;
     const size_t np = (size_t)bc->max_problems;
     if (hipMalloc(&b->arena, (size_t)bc->arena_bytes) != hipSuccess ||
         hipMalloc(&b->scal, sizeof(double) * SCAL_DOUBLES * np) != hipSuccess ||
@@ -287,12 +307,13 @@ int32_t cfs_batch_admit(cfs_batch* b, const cfs_config* cfg, const double* U, co
     const bool has_ib = ib != nullptr;
     const int n_ig = has_ib ? ib->n_ig : 0;
     if (has_ib && (n_ig <= 0 || !ib->image_xy || !ib->coef)) return CFS_E_ARG;
-    const int tiles_i = (cfg->nx + TI - 1) / TI, tiles_j = (cfg->ny + TJ - 1) / TJ;
+    const ShapeDims sd = shape_dims(b->bc.tile_shape);
+    const int tiles_i = (cfg->nx + sd.ti - 1) / sd.ti, tiles_j = (cfg->ny + sd.tj - 1) / sd.tj;
     std::vector<int> index;
     std::vector<uint8_t> tflags;
     if (has_ib) {
-        rc = build_ib_index(*cfg, tiles_i, tiles_j, ib->interp, ib->cell_type, n_ig, ib->ig_cells, ib->donors,
-                           ib->transp, index, tflags);
+        rc = build_ib_index(*cfg, b->bc.tile_shape, ib->interp, ib->cell_type, n_ig, ib->ig_cells, ib->donors,
+                            ib->transp, index, tflags);
         if (rc) return rc;
     }
     int p = -1;
@@ -302,7 +323,7 @@ int32_t cfs_batch_admit(cfs_batch* b, const cfs_config* cfg, const double* U, co
     long long used_tiles = 0;
     for (const Slot& s : b->slots)
         if (s.used) used_tiles += s.ntiles;
-    if (used_tiles + tiles_of(*cfg) > b->tiles_cap) return CFS_E_FULL;
+    if (used_tiles + tiles_of(b, *cfg) > b->tiles_cap) return CFS_E_FULL;
     const Layout L = layout_of(*cfg, n_ig, has_ib);
     size_t off = 0;
     if (!b->alloc.alloc(L.total, &off)) return CFS_E_FULL;
@@ -312,7 +333,7 @@ int32_t cfs_batch_admit(cfs_batch* b, const cfs_config* cfg, const double* U, co
     s.cfg = *cfg;
     s.off = off;
     s.bytes = L.total;
-    s.ntiles = (int)tiles_of(*cfg);
+    s.ntiles = (int)tiles_of(b, *cfg);
     ProblemDev& d = s.dev;
     char* base = b->arena + off;
     d.cfg = &s.cfg;
@@ -463,6 +484,45 @@ int32_t cfs_batch_capacity(const cfs_batch* b, int64_t* arena_bytes, int32_t* ma
     if (!b || !arena_bytes || !max_problems) return CFS_E_ARG;
     *arena_bytes = b->bc.arena_bytes;
     *max_problems = b->bc.max_problems;
+    return CFS_OK;
+}
+
+int32_t cfs_batch_autotune(const cfs_batch_config* bc, const cfs_request* sample, int32_t n, int32_t steps,
+                           int32_t* best, double* ms_per_step) {
+    if (!bc || !sample || n < 1 || steps < 1 || steps > 100000) return CFS_E_ARG;
+    int k_best = 0;
+    double t_best = 0.0;
+    for (int k = 0; k < NUM_SHAPES; ++k) {
+        cfs_batch_config bck = *bc;
+        bck.tile_shape = k;
+        cfs_batch* b = nullptr;
+        int rc = cfs_batch_create(&bck, nullptr, &b);
+        if (rc) return rc;
+        for (int r = 0; r < n && !rc; ++r) {
+            int32_t slot = 0;
+            rc = cfs_batch_admit(b, sample[r].cfg, sample[r].U, sample[r].ib, sample[r].cfl, 0.0, &slot);
+        }
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        float ms = 0.f;
+        if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = CFS_E_HIP;
+        if (!rc) rc = cfs_batch_step(b, 2);
+        if (!rc && hipEventRecord(e0, b->stream) != hipSuccess) rc = CFS_E_HIP;
+        if (!rc) rc = cfs_batch_step(b, steps);
+        if (!rc && (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = CFS_E_HIP;
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        cfs_batch_destroy(b);
+        if (rc) return rc;
+        const double t = (double)ms / steps;
+        if (ms_per_step) ms_per_step[k] = t;
+        if (k == 0 || t < t_best) {
+            k_best = k;
+            t_best = t;
+        }
+    }
+    if (best) *best = k_best;
     return CFS_OK;
 }
 

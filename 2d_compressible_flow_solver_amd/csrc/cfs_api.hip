@@ -50,6 +50,8 @@ struct cfs_solver {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     GraphKey graph_key;
+    int shape = 0;                              // tile shape of the stage kernels
+    uint8_t* tile_ib_shape[NUM_SHAPES] = {};    // per-tile IB flags of every shape
 };
 
 namespace {
@@ -64,23 +66,42 @@ void drop_graph(cfs_solver* s) {
     s->graph_key = GraphKey{};
 }
 
-template <typename R, bool VISC, bool LIMIT>
+template <typename R, bool VISC, bool LIMIT, typename Sh>
 void launch_stage_kernel(dim3 grid, hipStream_t stream, const StageArgs& a) {
-    hipLaunchKernelGGL((stage_kernel<R, VISC, LIMIT>), grid, dim3(BLOCK), 0, stream, a);
+    hipLaunchKernelGGL((stage_kernel<R, VISC, LIMIT, Sh>), grid, dim3(Sh::NT), 0, stream, a);
 }
 
-// the kernel variant of a configuration: precision, viscous terms, limiter
-void dispatch_stage(const cfs_config& c, dim3 grid, hipStream_t stream, const StageArgs& a) {
+template <typename R, bool VISC, bool LIMIT>
+void launch_shape(int shape, dim3 grid, hipStream_t stream, const StageArgs& a) {
+    if (VISC || shape == 0) launch_stage_kernel<R, VISC, LIMIT, Shape0>(grid, stream, a);
+    else if (shape == 1) launch_stage_kernel<R, false, LIMIT, Shape1>(grid, stream, a);
+    else launch_stage_kernel<R, false, LIMIT, Shape2>(grid, stream, a);
+}
+
+// the kernel variant of a configuration: precision, viscous terms, limiter,
+// tile shape (the viscous variant has the 15 x 16 shape only)
+void dispatch_stage(const cfs_config& c, int shape, dim3 grid, hipStream_t stream, const StageArgs& a) {
     const bool limit = c.positivity == 2;
     if (c.precision == CFS_PREC_FP64) {
-        if (c.viscous) launch_stage_kernel<double, true, false>(grid, stream, a);
-        else if (limit) launch_stage_kernel<double, false, true>(grid, stream, a);
-        else launch_stage_kernel<double, false, false>(grid, stream, a);
+        if (c.viscous) launch_shape<double, true, false>(0, grid, stream, a);
+        else if (limit) launch_shape<double, false, true>(shape, grid, stream, a);
+        else launch_shape<double, false, false>(shape, grid, stream, a);
     } else {
-        if (c.viscous) launch_stage_kernel<float, true, false>(grid, stream, a);
-        else if (limit) launch_stage_kernel<float, false, true>(grid, stream, a);
-        else launch_stage_kernel<float, false, false>(grid, stream, a);
+        if (c.viscous) launch_shape<float, true, false>(0, grid, stream, a);
+        else if (limit) launch_shape<float, false, true>(shape, grid, stream, a);
+        else launch_shape<float, false, false>(shape, grid, stream, a);
     }
+}
+
+dim3 stage_grid(const cfs_solver* s) { return dim3((unsigned)(8 * ((s->dev.tiles_i * s->dev.tiles_j + 7) / 8))); }
+
+// Makes tile shape k current: tile counts and the IB tile flags of that shape.
+void apply_shape(cfs_solver* s, int k) {
+    const ShapeDims d = shape_dims(k);
+    s->shape = k;
+    s->dev.tiles_i = (s->cfg.nx + d.ti - 1) / d.ti;
+    s->dev.tiles_j = (s->cfg.ny + d.tj - 1) / d.tj;
+    s->dev.tile_ib = s->tile_ib_shape[k];
 }
 
 // One RK stage of the global stage counter.
@@ -88,7 +109,7 @@ int launch_stage(cfs_solver* s, int counter, bool timed, const StageTime& tm) {
     const cfs_config& c = s->cfg;
     StageArgs a;
     const int next = fill_stage_args(a, s->dev, rk_substage(c.rk, counter), s->step_parity, s->parity, tm);
-    const dim3 grid((unsigned)(8 * ((s->dev.tiles_i * s->dev.tiles_j + 7) / 8)));
+    const dim3 grid = stage_grid(s);
     if (timed) {
         hipEvent_t e0, e1;
         HIP_OK(hipEventCreate(&e0));
@@ -97,7 +118,7 @@ int launch_stage(cfs_solver* s, int counter, bool timed, const StageTime& tm) {
         s->ev.push_back(e1);
         HIP_OK(hipEventRecord(e0, s->stream));
     }
-    dispatch_stage(c, grid, s->stream, a);
+    dispatch_stage(c, s->shape, grid, s->stream, a);
     HIP_OK(hipGetLastError());
     if (timed) HIP_OK(hipEventRecord(s->ev.back(), s->stream));
     s->parity ^= 1;
@@ -131,7 +152,10 @@ void free_ib(cfs_solver* s) {
     (void)hipFree(d.ig_ldl);
     (void)hipFree(d.ig_transp);
     (void)hipFree(d.ig_index);
-    (void)hipFree(d.tile_ib);
+    for (int k = 0; k < NUM_SHAPES; ++k) {
+        (void)hipFree(s->tile_ib_shape[k]);
+        s->tile_ib_shape[k] = nullptr;
+    }
     d.ig_cells = d.ig_donors = d.ig_transp = d.ig_index = nullptr;
     d.ig_xy = d.ig_ldl = nullptr;
     d.tile_ib = nullptr;
@@ -177,8 +201,7 @@ int32_t cfs_create(const cfs_config* cfg, void* hip_stream, cfs_solver** out) {
     ProblemDev& d = s->dev;
     d.cfg = &s->cfg;
     d.plane = (long long)cfg->nx * cfg->ny;
-    d.tiles_i = (cfg->nx + TI - 1) / TI;
-    d.tiles_j = (cfg->ny + TJ - 1) / TJ;
+    apply_shape(s, 0);
     if (hip_stream) {
         s->stream = (hipStream_t)hip_stream;
     } else {
@@ -261,10 +284,10 @@ int32_t cfs_set_immersed(cfs_solver* s, int32_t interp, const uint8_t* cell_type
     if (n_ig > 0 && (!image_xy || !ldlt)) return CFS_E_ARG;
     ProblemDev& d = s->dev;
     std::vector<int> index;
-    std::vector<uint8_t> tiles;
-    int rc = build_ib_index(s->cfg, d.tiles_i, d.tiles_j, interp, cell_type, n_ig, ig_cells, donors, transp, index,
-                            tiles);
+    std::vector<uint8_t> tiles[NUM_SHAPES];
+    int rc = build_ib_index(s->cfg, 0, interp, cell_type, n_ig, ig_cells, donors, transp, index, tiles[0]);
     if (rc) return rc;
+    for (int k = 1; k < NUM_SHAPES; ++k) ib_tile_flags(s->cfg, k, n_ig, ig_cells, tiles[k]);
     HIP_OK(hipStreamSynchronize(s->stream));
     drop_graph(s);
     free_ib(s);
@@ -276,21 +299,27 @@ int32_t cfs_set_immersed(cfs_solver* s, int32_t interp, const uint8_t* cell_type
             hipMalloc(&d.ig_xy, sizeof(double) * 2 * n_ig) != hipSuccess ||
             hipMalloc(&d.ig_ldl, sizeof(double) * 10 * n_ig) != hipSuccess ||
             hipMalloc(&d.ig_transp, sizeof(int) * 4 * n_ig) != hipSuccess ||
-            hipMalloc(&d.ig_index, sizeof(int) * (size_t)d.plane) != hipSuccess ||
-            hipMalloc(&d.tile_ib, tiles.size()) != hipSuccess) {
+            hipMalloc(&d.ig_index, sizeof(int) * (size_t)d.plane) != hipSuccess) {
             free_ib(s);
             return CFS_E_OOM;
         }
+        for (int k = 0; k < NUM_SHAPES; ++k)
+            if (hipMalloc(&s->tile_ib_shape[k], tiles[k].size()) != hipSuccess) {
+                free_ib(s);
+                return CFS_E_OOM;
+            }
         HIP_OK(hipMemcpy(d.ig_cells, ig_cells, sizeof(int) * n_ig, hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(d.ig_donors, donors, sizeof(int) * 4 * n_ig, hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(d.ig_xy, image_xy, sizeof(double) * 2 * n_ig, hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(d.ig_ldl, ldlt, sizeof(double) * 10 * n_ig, hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(d.ig_transp, transp, sizeof(int) * 4 * n_ig, hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(d.ig_index, index.data(), sizeof(int) * (size_t)d.plane, hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy(d.tile_ib, tiles.data(), tiles.size(), hipMemcpyHostToDevice));
+        for (int k = 0; k < NUM_SHAPES; ++k)
+            HIP_OK(hipMemcpy(s->tile_ib_shape[k], tiles[k].data(), tiles[k].size(), hipMemcpyHostToDevice));
     }
     d.n_ig = n_ig;
     d.ib_interp = interp;
+    apply_shape(s, s->shape);
     return CFS_OK;
 }
 
@@ -523,11 +552,87 @@ int32_t cfs_last_rhs(cfs_solver* s, double* R) {
     int* cnt = reinterpret_cast<int*>(s->rhs + 4 * (size_t)s->dev.plane);
     a.exact_count = cnt;
     a.exact_other = cnt + 1;
-    const dim3 grid((unsigned)(8 * ((s->dev.tiles_i * s->dev.tiles_j + 7) / 8)));
-    dispatch_stage(c, grid, s->stream, a);
+    dispatch_stage(c, s->shape, stage_grid(s), s->stream, a);
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(R, s->rhs, bytes, hipMemcpyDeviceToHost, s->stream));
     HIP_OK(hipStreamSynchronize(s->stream));
+    return CFS_OK;
+}
+
+int32_t cfs_tile_shape_info(int32_t shape, int32_t* ti, int32_t* tj, int32_t* threads) {
+    if (shape < 0 || shape >= NUM_SHAPES) return CFS_E_ARG;
+    const ShapeDims d = shape_dims(shape);
+    if (ti) *ti = d.ti;
+    if (tj) *tj = d.tj;
+    if (threads) *threads = d.nt;
+    return CFS_OK;
+}
+
+int32_t cfs_set_tile_shape(cfs_solver* s, int32_t shape) {
+    if (!s || shape < 0 || shape >= NUM_SHAPES) return CFS_E_ARG;
+    if (s->cfg.viscous && shape != 0) return CFS_E_UNSUPPORTED;
+    HIP_OK(hipStreamSynchronize(s->stream));
+    if (shape != s->shape) drop_graph(s);
+    apply_shape(s, shape);
+    return CFS_OK;
+}
+
+int32_t cfs_get_tile_shape(const cfs_solver* s, int32_t* shape) {
+    if (!s || !shape) return CFS_E_ARG;
+    *shape = s->shape;
+    return CFS_OK;
+}
+
+int32_t cfs_autotune(cfs_solver* s, int32_t reps, int32_t* best, double* ms) {
+    if (!s || reps < 1 || reps > 10000) return CFS_E_ARG;
+    // the timed stage writes the spare plane-set 1, which holds no stage data
+    // only at a step boundary
+    if (rk_substage(s->cfg.rk, s->counter) != 0) return CFS_E_STATE;
+    const int nshape = s->cfg.viscous ? 1 : NUM_SHAPES;
+    const int shape0 = s->shape;
+    int* cnt = nullptr;
+    if (hipMalloc(&cnt, 2 * sizeof(int)) != hipSuccess) return CFS_E_OOM;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = CFS_OK;
+    double t[NUM_SHAPES];
+    int k_best = 0;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = CFS_E_HIP;
+    if (!rc) rc = set_dt_device(s, 0.0);
+    // sub-stage 0 of the current state, fixed dt = 0, into plane-set 1: the
+    // stage kernel's whole work (tile load with the boundary and IB fills,
+    // faces, update, stores) without changing the solve
+    for (int k = 0; k < nshape && !rc; ++k) {
+        apply_shape(s, k);
+        StageArgs a;
+        fill_stage_args(a, s->dev, 0, s->step_parity, 0, StageTime{});
+        a.exact_count = cnt;
+        a.exact_other = cnt + 1;
+        const dim3 grid = stage_grid(s);
+        for (int w = 0; w < 2; ++w) dispatch_stage(s->cfg, k, grid, s->stream, a);
+        if (hipEventRecord(e0, s->stream) != hipSuccess) { rc = CFS_E_HIP; break; }
+        for (int n = 0; n < reps; ++n) dispatch_stage(s->cfg, k, grid, s->stream, a);
+        float msk = 0.f;
+        if (hipGetLastError() != hipSuccess || hipEventRecord(e1, s->stream) != hipSuccess ||
+            hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&msk, e0, e1) != hipSuccess) {
+            rc = CFS_E_HIP;
+            break;
+        }
+        t[k] = msk / reps;
+        if (ms) ms[k] = t[k];
+        if (k == 0 || t[k] < t[k_best]) k_best = k;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(s->stream);
+    (void)hipFree(cnt);
+    if (rc) {
+        apply_shape(s, shape0);
+        return rc;
+    }
+    for (int k = nshape; k < NUM_SHAPES && ms; ++k) ms[k] = -1.0;
+    drop_graph(s);
+    apply_shape(s, k_best);
+    if (best) *best = k_best;
     return CFS_OK;
 }
 

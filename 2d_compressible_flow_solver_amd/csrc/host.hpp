@@ -176,10 +176,45 @@ inline StageTime cfl_stage_time(int rk, int k, double cfl, double t_end) {
     return tm;
 }
 
+// Tile shapes of the stage kernels (kernels.hpp), by index.
+struct ShapeDims {
+    int ti, tj, nt;
+};
+inline ShapeDims shape_dims(int k) {
+    if (k == 1) return ShapeDims{Shape1::TI, Shape1::TJ, Shape1::NT};
+    if (k == 2) return ShapeDims{Shape2::TI, Shape2::TJ, Shape2::NT};
+    return ShapeDims{Shape0::TI, Shape0::TJ, Shape0::NT};
+}
+inline long long tiles_for(const cfs_config& c, int k) {
+    const ShapeDims d = shape_dims(k);
+    return (long long)((c.nx + d.ti - 1) / d.ti) * ((c.ny + d.tj - 1) / d.tj);
+}
+
+// Per-tile flag of tile shape k: "the staged region (tile + 3-cell halo,
+// periodic wrap) holds an immersed ghost".
+inline void ib_tile_flags(const cfs_config& c, int k, int n_ig, const int32_t* ig_cells,
+                          std::vector<uint8_t>& tiles) {
+    const ShapeDims d = shape_dims(k);
+    const int nx = c.nx, ny = c.ny;
+    const int tiles_j = (ny + d.tj - 1) / d.tj;
+    tiles.assign((size_t)tiles_for(c, k), 0);
+    for (int n = 0; n < n_ig; ++n) {
+        const int gi = ig_cells[n] / ny, gj = ig_cells[n] % ny;
+        for (int di = -HALO; di <= HALO; ++di)
+            for (int dj = -HALO; dj <= HALO; ++dj) {
+                int a = gi + di, b = gj + dj;
+                if (c.bc[0] == CFS_BC_PERIODIC) a = (a % nx + nx) % nx;
+                if (c.bc[2] == CFS_BC_PERIODIC) b = (b % ny + ny) % ny;
+                if (a < 0 || a >= nx || b < 0 || b >= ny) continue;
+                tiles[(size_t)(a / d.ti) * tiles_j + b / d.tj] = 1;
+            }
+    }
+}
+
 // Checks immersed-boundary tables against the cell types and builds the
-// per-cell row index (-1 off the ghosts) and the per-tile flag "the staged
-// region (tile + 3-cell halo, periodic wrap) holds an immersed ghost".
-inline int build_ib_index(const cfs_config& c, int tiles_i, int tiles_j, int interp, const uint8_t* cell_type,
+// per-cell row index (-1 off the ghosts) and the per-tile flags of tile shape
+// `shape` (ib_tile_flags).
+inline int build_ib_index(const cfs_config& c, int shape, int interp, const uint8_t* cell_type,
                           int n_ig, const int32_t* ig_cells, const int32_t* donors, const int32_t* transp,
                           std::vector<int>& index, std::vector<uint8_t>& tiles) {
     if (!cell_type || n_ig < 0) return CFS_E_ARG;
@@ -188,7 +223,6 @@ inline int build_ib_index(const cfs_config& c, int tiles_i, int tiles_j, int int
     const int nx = c.nx, ny = c.ny;
     const long long plane = (long long)nx * ny;
     index.assign((size_t)plane, -1);
-    tiles.assign((size_t)tiles_i * tiles_j, 0);
     for (long long k = 0; k < plane; ++k)
         if (cell_type[k] > CFS_CELL_IMMERSED_GHOST) return CFS_E_ARG;
     for (int k = 0; k < n_ig; ++k) {
@@ -201,18 +235,10 @@ inline int build_ib_index(const cfs_config& c, int tiles_i, int tiles_j, int int
             const int t = transp[4 * k + d];
             if (interp == CFS_IB_REFERENCE && (t < d || t > 3)) return CFS_E_ARG;
         }
-        const int gi = ig_cells[k] / ny, gj = ig_cells[k] % ny;
-        for (int di = -HALO; di <= HALO; ++di)
-            for (int dj = -HALO; dj <= HALO; ++dj) {
-                int a = gi + di, b = gj + dj;
-                if (c.bc[0] == CFS_BC_PERIODIC) a = (a % nx + nx) % nx;
-                if (c.bc[2] == CFS_BC_PERIODIC) b = (b % ny + ny) % ny;
-                if (a < 0 || a >= nx || b < 0 || b >= ny) continue;
-                tiles[(size_t)(a / TI) * tiles_j + b / TJ] = 1;
-            }
     }
     for (long long k = 0; k < plane; ++k)
         if (cell_type[k] == CFS_CELL_IMMERSED_GHOST && index[k] < 0) return CFS_E_ARG;   // ghost without a row
+    ib_tile_flags(c, shape, n_ig, ig_cells, tiles);
     return CFS_OK;
 }
 

@@ -5,7 +5,8 @@
 // optional uint8 cell-type plane [nx][ny]; immersed-boundary tables.
 //
 // Stage kernel (one launch per RK stage): a 256-thread workgroup owns a
-// TI x TJ = 15 x 16 cell tile.  It stages the tile plus a 3-cell halo
+// TI x TJ cell tile (15 x 16 with 256 threads by default, larger shapes
+// below).  It stages the tile plus a 3-cell halo
 // (the WENO5 stencil reach, minus the unused corners) in LDS, applying the
 // physical boundary on the fly (one ghost cell per boundary face, as
 // source/boundary/*Boundary.cpp; periodic wrap), then
@@ -32,14 +33,22 @@
 
 namespace cfs {
 
-constexpr int TI = 15;             // tile rows (x / i direction)
-constexpr int TJ = 16;             // tile columns (y / j direction, contiguous)
 constexpr int HALO = 3;            // WENO5 reach: stencil offsets -3..+2
-constexpr int SR = TI + 2 * HALO;  // 21 LDS rows
-constexpr int SC = TJ + 2 * HALO;  // 22 LDS columns
-constexpr int NXF = (TI + 1) * TJ; // 256 x faces per tile
-constexpr int NYF = TI * (TJ + 1); // 255 y faces per tile
-constexpr int BLOCK = 256;
+
+// Tile shape: a workgroup of NT threads owns TI x TJ cells (TI rows along x /
+// i, TJ columns along y / j, j contiguous in memory) and stages SR x SC cells
+// (the tile plus the halo) in LDS.  Its (TI + 1) TJ x-faces and TI (TJ + 1)
+// y-faces take two rounds of NT threads; every cell update one thread.
+template <int TI_, int TJ_, int NT_> struct TileShape {
+    static constexpr int TI = TI_, TJ = TJ_, NT = NT_;
+    static constexpr int SR = TI + 2 * HALO, SC = TJ + 2 * HALO;
+    static constexpr int NXF = (TI + 1) * TJ, NYF = TI * (TJ + 1);
+    static_assert(NXF + NYF <= 2 * NT && TI * TJ <= NT && NT % 64 == 0, "two face rounds, one cell per thread");
+};
+using Shape0 = TileShape<15, 16, 256>;    // 240 cells, 511 faces (256 + 255)
+using Shape1 = TileShape<15, 32, 512>;    // 480 cells, 1007 faces (512 + 495)
+using Shape2 = TileShape<31, 32, 1024>;   // 992 cells, 2047 faces (1024 + 1023)
+constexpr int NUM_SHAPES = 3;
 
 enum { BC_NEUMANN = 0, BC_DIRICHLET = 1, BC_PERIODIC = 2 };
 
@@ -91,9 +100,10 @@ struct StageArgs {
 
 // LDS of the viscous variant: primitive u, v, T of every staged cell and the
 // cell gradients d/dy (x-face cross terms) and d/dx (y-face cross terms) of u, v
-template <typename R> struct ViscLDS {
-    R pu[SR * SC], pv[SR * SC], pT[SR * SC];
-    R gyu[SR * SC], gyv[SR * SC], gxu[SR * SC], gxv[SR * SC];
+template <typename R, typename Sh> struct ViscLDS {
+    static constexpr int N = Sh::SR * Sh::SC;
+    R pu[N], pv[N], pT[N];
+    R gyu[N], gyv[N], gxu[N], gxv[N];
 };
 
 // ---------------------------------------------------------------------------
@@ -163,6 +173,8 @@ template <> struct WOps<double, true> {
         return __builtin_fma(__builtin_fma(-b, q, a), r, q);
     }
     __device__ static double div(double a, double b) { return div_r(a, b, rcp(b)); }
+    // x - k*y for k = 2, 4: k*y is exact below 2^1021, so one fma rounds once as well
+    __device__ static double sub_pow2(double x, double k, double y) { return __builtin_fma(-k, y, x); }
 };
 template <> struct WOps<double, false> {
     __device__ static double d3(double x) { return x / 3.0; }
@@ -170,6 +182,7 @@ template <> struct WOps<double, false> {
     __device__ static double rcp(double b) { return b; }
     __device__ static double div_r(double a, double b, double) { return a / b; }
     __device__ static double div(double a, double b) { return a / b; }
+    __device__ static double sub_pow2(double x, double k, double y) { return x - k * y; }
 };
 template <bool FAST> struct WOps<float, FAST> {
     __device__ static float d3(float x) { return x * (1.0f / 3.0f); }
@@ -177,6 +190,7 @@ template <bool FAST> struct WOps<float, FAST> {
     __device__ static float rcp(float b) { return __builtin_amdgcn_rcpf(b); }
     __device__ static float div_r(float a, float, float r) { return a * r; }
     __device__ static float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    __device__ static float sub_pow2(float x, float k, float y) { return x - k * y; }
 };
 
 // range of the staged values for which WOps<double, true> is exact: every
@@ -194,14 +208,14 @@ template <typename R, bool FAST>
 __device__ __forceinline__ R weno5(R v0, R v1, R v2, R v3, R v4) {
     using C = K<R>;
     using D = WOps<R, FAST>;
-    const R s0 = (v0 - R(2) * v1) + v2;
-    const R t0 = (v0 - R(4) * v1) + R(3) * v2;
+    const R s0 = D::sub_pow2(v0, R(2), v1) + v2;
+    const R t0 = D::sub_pow2(v0, R(4), v1) + R(3) * v2;
     const R b0 = C::c1312 * (s0 * s0) + C::q * (t0 * t0);
-    const R s1 = (v1 - R(2) * v2) + v3;
+    const R s1 = D::sub_pow2(v1, R(2), v2) + v3;
     const R t1 = v1 - v3;
     const R b1 = C::c1312 * (s1 * s1) + C::q * (t1 * t1);
-    const R s2 = (v2 - R(2) * v3) + v4;
-    const R t2 = (R(3) * v2 - R(4) * v3) + v4;
+    const R s2 = D::sub_pow2(v2, R(2), v3) + v4;
+    const R t2 = D::sub_pow2(R(3) * v2, R(4), v3) + v4;
     const R b2 = C::c1312 * (s2 * s2) + C::q * (t2 * t2);
     const R e0 = C::eps + b0, e1 = C::eps + b1, e2 = C::eps + b2;
     const R a0 = D::div(C::d0, e0 * e0);
@@ -454,8 +468,8 @@ __device__ __forceinline__ void block_max_to(double m, unsigned long long* acc, 
 // SSP Runge-Kutta stages (RK2, and RK3 except the reference's last stage) the
 // new state keeps rho, p > 0.  All in the face-normal frame.
 
-template <typename R> struct LimitLDS {
-    R sx[SR * SC], sy[SR * SC];   // |u| + c and |v| + c of every staged cell
+template <typename R, typename Sh> struct LimitLDS {
+    R sx[Sh::SR * Sh::SC], sy[Sh::SR * Sh::SC];   // |u| + c and |v| + c of every staged cell
 };
 
 template <typename R> __device__ __forceinline__ R maxr(R x, R y) { return (x < y) ? y : x; }
@@ -504,8 +518,8 @@ __device__ __forceinline__ void euler_flux_n(R r, R n, R t, R E, R gm1, R F[4]) 
 
 // cL, cR: LDS cells left and right of the face; dn, dt: LDS offsets of the
 // neighbours along the normal and the tangent; ln, lt: dt/h along them
-template <typename R>
-__device__ __forceinline__ void limit_face(const R* __restrict__ S, const LimitLDS<R>& Lm, int cL, int cR, int nrm,
+template <typename R, typename Sh>
+__device__ __forceinline__ void limit_face(const R* __restrict__ S, const LimitLDS<R, Sh>& Lm, int cL, int cR, int nrm,
                                            int tng, int dn, int dtg, R ln, R lt, R g, R fl[4]) {
     using C = K<R>;
     const R* sn = (nrm == 1) ? Lm.sx : Lm.sy;
@@ -584,8 +598,8 @@ template <> struct VT<float> {
 
 // net viscous flux (0, normal-momentum, tangential-momentum, energy) of a face
 // whose 6 stencil cells have LDS cell indices cs[k]
-template <typename R>
-__device__ __forceinline__ void viscous_flux(const ViscLDS<R>& V, const int cs[6], bool xface, R h, R mu, R kappa,
+template <typename R, typename Sh>
+__device__ __forceinline__ void viscous_flux(const ViscLDS<R, Sh>& V, const int cs[6], bool xface, R h, R mu, R kappa,
                                              R out[3]) {
     const R* un = xface ? V.pu : V.pv;
     const R* ut = xface ? V.pv : V.pu;
@@ -625,9 +639,10 @@ __device__ __forceinline__ void viscous_flux(const ViscLDS<R>& V, const int cs[6
 
 // Phase 1 of a tile: its 511 faces, fluxes to LDS.  FAST selects the WENO5
 // division forms (WOps); the caller picks it per tile.
-template <typename R, bool VISC, bool LIMIT, bool FAST>
+template <typename R, bool VISC, bool LIMIT, bool FAST, typename Sh>
 __device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, double dt, const R* S, R* F,
-                                           const ViscLDS<R>* V, const LimitLDS<R>* Lm) {
+                                           const ViscLDS<R, Sh>* V, const LimitLDS<R, Sh>* Lm) {
+    constexpr int TJ = Sh::TJ, SC = Sh::SC, NXF = Sh::NXF, NYF = Sh::NYF, BLOCK = Sh::NT;
     const int tid = threadIdx.x;
     const bool px = a.bc[0] == BC_PERIODIC, py = a.bc[2] == BC_PERIODIC;
     const R g = (R)a.gamma;
@@ -666,7 +681,7 @@ __device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, d
         if constexpr (LIMIT) {
             if (a.positivity == 2) {
                 const bool xf = s < NXF;
-                limit_face<R>(S, *Lm, cell[2] >> 2, cell[3] >> 2, nrm, tng, xf ? SC : 1, xf ? 1 : SC,
+                limit_face<R, Sh>(S, *Lm, cell[2] >> 2, cell[3] >> 2, nrm, tng, xf ? SC : 1, xf ? 1 : SC,
                               xf ? lam_x : lam_y, xf ? lam_y : lam_x, g, fl);
             }
         }
@@ -676,7 +691,7 @@ __device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, d
             for (int k = 0; k < 6; ++k) cs[k] = cell[k] >> 2;
             const bool xface = s < NXF;
             R fv[3];
-            viscous_flux<R>(*V, cs, xface, xface ? (R)a.dx : (R)a.dy, (R)a.mu, (R)a.kappa, fv);
+            viscous_flux<R, Sh>(*V, cs, xface, xface ? (R)a.dx : (R)a.dy, (R)a.mu, (R)a.kappa, fv);
             fl[1] = fl[1] - fv[0];
             fl[2] = fl[2] - fv[1];
             fl[3] = fl[3] - fv[2];
@@ -688,9 +703,10 @@ __device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, d
     }
 }
 
-template <typename R, bool VISC, bool LIMIT>
+template <typename R, bool VISC, bool LIMIT, typename Sh>
 __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R* F, double* wmax,
-                                           ViscLDS<R>* V, LimitLDS<R>* Lm) {
+                                           ViscLDS<R, Sh>* V, LimitLDS<R, Sh>* Lm) {
+    constexpr int TI = Sh::TI, TJ = Sh::TJ, SR = Sh::SR, SC = Sh::SC, NXF = Sh::NXF, BLOCK = Sh::NT;
     const int tid = threadIdx.x;
     const int ti = tile / a.tiles_j;
     const int tj = tile - ti * a.tiles_j;
@@ -827,9 +843,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
 
     // ---- faces: the exact-division form on tiles outside the fast range ----
     if (sizeof(R) == 8 && tile_unsafe)
-        tile_faces<R, VISC, LIMIT, false>(a, i0, j0, dt, S, F, V, Lm);
+        tile_faces<R, VISC, LIMIT, false, Sh>(a, i0, j0, dt, S, F, V, Lm);
     else
-        tile_faces<R, VISC, LIMIT, true>(a, i0, j0, dt, S, F, V, Lm);
+        tile_faces<R, VISC, LIMIT, true, Sh>(a, i0, j0, dt, S, F, V, Lm);
     __syncthreads();
 
     // ---- cell update: flux divergence (Cell::comRHS, Cell.cpp:110-125) and the
@@ -881,32 +897,39 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
     if (a.fuse_cfl) block_max_to(smax, a.acc_out, wmax);
 }
 
-// The stage kernel: one workgroup per tile.  FP64: at most 96 VGPRs so that
-// five workgroups (20 waves) share a CU; FP32: six (the most without spills).
-// The viscous variant's LDS (57 KB in FP64) allows two workgroups per CU.
-// The limiter variant adds 7.4 KB (FP64) of LDS: four per CU.
+// Workgroups per CU the kernels are compiled for (VGPR budget): waves per CU
+// of 20 (FP64), 24 (FP32), 16 with the limiter (its LDS), 8 viscous (57 KB of
+// LDS per 256 threads in FP64), divided by the shape's waves per workgroup.
+template <typename R, bool VISC, bool LIMIT, typename Sh> struct Occupancy {
+    static constexpr int waves = VISC ? 8 : (LIMIT ? 16 : (sizeof(R) == 8 ? 20 : 24));
+    static constexpr int blocks = waves / (Sh::NT / 64) > 0 ? waves / (Sh::NT / 64) : 1;
+};
+
+// The stage kernel: one workgroup per tile.  With 15 x 16 tiles: FP64 at
+// most 96 VGPRs so that five workgroups (20 waves) share a CU; FP32 six; the
+// viscous variant's LDS (57 KB in FP64) allows two, the limiter's (38 KB) four.
 //
 // Tile order: workgroups are dispatched round-robin over the 8 XCDs
 // (workgroup b to XCD b % 8), each with its own L2.  The grid is 8 x ceil(n/8)
 // and workgroup b takes tile (b % 8) * ceil(n/8) + b / 8, so each XCD sweeps
 // one contiguous band of tiles and neighbouring tiles, which share halo
 // cells, are staged through the same L2.
-template <typename R, bool VISC, bool LIMIT>
-__global__ __launch_bounds__(BLOCK, VISC ? 2 : (LIMIT ? 4 : (sizeof(R) == 8 ? 5 : 6))) void stage_kernel(StageArgs a) {
-    __shared__ R S[SR * SC * 4];
-    __shared__ R F[(NXF + NYF) * 4];
-    __shared__ double wmax[BLOCK / 64];
+template <typename R, bool VISC, bool LIMIT, typename Sh>
+__global__ __launch_bounds__(Sh::NT, (Occupancy<R, VISC, LIMIT, Sh>::blocks)) void stage_kernel(StageArgs a) {
+    __shared__ R S[Sh::SR * Sh::SC * 4];
+    __shared__ R F[(Sh::NXF + Sh::NYF) * 4];
+    __shared__ double wmax[Sh::NT / 64];
     const int chunk = (a.ntiles + 7) >> 3;
     const int tile = (int)(blockIdx.x & 7) * chunk + (int)(blockIdx.x >> 3);
     if (tile >= a.ntiles) return;
     if constexpr (VISC) {
-        __shared__ ViscLDS<R> V;
-        stage_tile<R, true, false>(a, tile, S, F, wmax, &V, nullptr);
+        __shared__ ViscLDS<R, Sh> V;
+        stage_tile<R, true, false, Sh>(a, tile, S, F, wmax, &V, nullptr);
     } else if constexpr (LIMIT) {
-        __shared__ LimitLDS<R> Lm;
-        stage_tile<R, false, true>(a, tile, S, F, wmax, nullptr, &Lm);
+        __shared__ LimitLDS<R, Sh> Lm;
+        stage_tile<R, false, true, Sh>(a, tile, S, F, wmax, nullptr, &Lm);
     } else {
-        stage_tile<R, false, false>(a, tile, S, F, wmax, nullptr, nullptr);
+        stage_tile<R, false, false, Sh>(a, tile, S, F, wmax, nullptr, nullptr);
     }
 }
 
@@ -917,14 +940,14 @@ __global__ __launch_bounds__(BLOCK, VISC ? 2 : (LIMIT ? 4 : (sizeof(R) == 8 ? 5 
 // halo cells are staged through the same L2.  Every problem's arithmetic is
 // the single-solve kernel's, so a problem's result does not depend on what
 // else is in the batch.  The grid size is a multiple of 8.
-template <typename R, bool LIMIT>
-__global__ __launch_bounds__(BLOCK, LIMIT ? 4 : (sizeof(R) == 8 ? 5 : 6)) void batch_stage_kernel(
+template <typename R, bool LIMIT, typename Sh>
+__global__ __launch_bounds__(Sh::NT, (Occupancy<R, false, LIMIT, Sh>::blocks)) void batch_stage_kernel(
     const StageArgs* __restrict__ descs, const int2* __restrict__ tiles, const int* __restrict__ ntiles, int parity,
     int sub) {
-    __shared__ R S[SR * SC * 4];
-    __shared__ R F[(NXF + NYF) * 4];
-    __shared__ double wmax[BLOCK / 64];
-    using LimitT = typename std::conditional<LIMIT, LimitLDS<R>, char>::type;
+    __shared__ R S[Sh::SR * Sh::SC * 4];
+    __shared__ R F[(Sh::NXF + Sh::NYF) * 4];
+    __shared__ double wmax[Sh::NT / 64];
+    using LimitT = typename std::conditional<LIMIT, LimitLDS<R, Sh>, char>::type;
     __shared__ LimitT Lm;
     const long long n = *ntiles;
     const int per = (int)(gridDim.x >> 3);
@@ -933,9 +956,10 @@ __global__ __launch_bounds__(BLOCK, LIMIT ? 4 : (sizeof(R) == 8 ? 5 : 6)) void b
     for (int t = lo + l; t < hi; t += per) {
         const int2 pt = tiles[t];
         if constexpr (LIMIT)
-            stage_tile<R, false, true>(descs[(pt.x * 2 + parity) * 3 + sub], pt.y, S, F, wmax, nullptr, &Lm);
+            stage_tile<R, false, true, Sh>(descs[(pt.x * 2 + parity) * 3 + sub], pt.y, S, F, wmax, nullptr, &Lm);
         else
-            stage_tile<R, false, false>(descs[(pt.x * 2 + parity) * 3 + sub], pt.y, S, F, wmax, nullptr, nullptr);
+            stage_tile<R, false, false, Sh>(descs[(pt.x * 2 + parity) * 3 + sub], pt.y, S, F, wmax, nullptr,
+                                            nullptr);
         __syncthreads();   // the next tile reuses S, F and wmax
     }
 }

@@ -53,6 +53,20 @@ REFERENCE_INFLOW = (1.0, 1.0, 0.0, 1.0)     # Cylinder.cpp:5-16
 BYTES_PER_CELL_STAGE = 97.0
 
 
+def kernel_name(prec, limiter, shape):
+    ti, tj, nt = cfs.tile_shape_info(shape)
+    return "stage_kernel<%s, false, %s, TileShape<%d, %d, %d>>" % (
+        "double" if prec == cfs.PREC_FP64 else "float", "true" if limiter else "false", ti, tj, nt)
+
+
+def shape_report(r):
+    ti, tj, nt = cfs.tile_shape_info(r["shape"])
+    out = {"chosen": r["shape"], "tile": [ti, tj], "threads": nt}
+    if r["shape_ms"] is not None:
+        out["autotune_ms_per_launch"] = [round(x, 5) for x in r["shape_ms"]]
+    return out
+
+
 def traffic_from_profiles():
     """HBM bytes per stage-kernel launch measured with rocprofv3 --pmc
     (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), from the
@@ -75,28 +89,37 @@ def setup(precision, reference_scheme=False):
     return cfg, ib, U
 
 
-def run_gpu(precision, steps, warmup, reference_scheme=False):
+def run_gpu(precision, steps, warmup, reference_scheme=False, tile_shape=None):
+    """tile_shape None: the autotuner picks it on the initial state."""
     cfg, ib, U = setup(precision, reference_scheme)
     s = cfs.Solver(cfg)
     s.set_immersed(ib)
     s.set_state(U)
+    if tile_shape is None:
+        shape, shape_ms = s.autotune(reps=20)
+    else:
+        s.set_tile_shape(tile_shape)
+        shape, shape_ms = tile_shape, None
+    tw = 0.0
     if warmup:
-        s.advance(warmup, dt=0.0, cfl=CFL)
+        tw, _ = s.advance(warmup, dt=0.0, cfl=CFL)
     t0 = time.perf_counter()
-    t, done = s.advance(steps, dt=0.0, cfl=CFL)
+    t, done = s.advance(steps, dt=0.0, cfl=CFL, t=tw)
     wall = time.perf_counter() - t0
     # stage-kernel duration: a separate pass with HIP events around every
     # stage kernel (the events add queue latency, so not in the timed window)
     s.set_kernel_timing(True)
-    s.advance(min(steps, 40), dt=0.0, cfl=CFL)
+    t_end, _ = s.advance(min(steps, 40), dt=0.0, cfl=CFL, t=t)
     kms = s.last_kernel_ms()
     V = s.get_state()
     ok = bool(np.all(np.isfinite(V)))
     cells = NX * NY
     stages = 3 * done
     s.close()
+    fluid = ib["cell_type"].reshape(NX, NY) == cfs.CELL_FLUID
     return dict(wall=wall, steps=done, mcups=cells * stages / wall / 1e6, kernel_ms=kms,
-                ms_per_step=1e3 * wall / max(done, 1), finite=ok, t=t)
+                ms_per_step=1e3 * wall / max(done, 1), finite=ok, t=t, t_end=t_end,
+                total_steps=warmup + done + min(steps, 40), rho=V[0][fluid], shape=shape, shape_ms=shape_ms)
 
 
 def run_cpu_baseline(budget_s=12.0):
@@ -138,7 +161,14 @@ def run_batch(precision, count, poll_every=20, seed=BATCH_SEED, max_steps=20000)
     total_cells = sum(c.nx * c.ny for c, _ in probs)
     arena = int(sum(cfs.Batch.footprint(c) for c, _ in probs) * 1.05) + (64 << 20)
     res = {}
-    with cfs.Batch(rk=2, precision=precision, max_problems=count, arena_bytes=arena, limiter=True) as b:
+    # tile shape: the autotuner on a sample of the workload (its first 64 problems)
+    sample = probs[:64]
+    shape, shape_ms = cfs.batch_autotune(sample, rk=2, precision=precision, limiter=True, steps=6,
+                                         arena_bytes=int(sum(cfs.Batch.footprint(c) for c, _ in sample) * 1.05)
+                                         + (64 << 20))
+    print("batch autotune: shape %d, ms per step %s" % (shape, shape_ms), file=sys.stderr, flush=True)
+    with cfs.Batch(rk=2, precision=precision, max_problems=count, arena_bytes=arena, limiter=True,
+                   tile_shape=shape) as b:
         t0 = time.perf_counter()
         slot_of = {}
         for k, (cfg, U) in enumerate(probs):
@@ -178,7 +208,8 @@ def run_batch(precision, count, poll_every=20, seed=BATCH_SEED, max_steps=20000)
     return dict(problems=count, cells=total_cells, cell_stages=cell_stages, wall=wall, admit_s=t_admit,
                 mcups=cell_stages / wall / 1e6, solves_per_s=count / wall, failed=failed, stalled=stalled,
                 seq_wall=seq_wall, seq_mcups=cell_stages / seq_wall / 1e6, seq_solves_per_s=count / seq_wall,
-                steps_min=min(r[1] for r in res.values()), steps_max=max(r[1] for r in res.values()))
+                steps_min=min(r[1] for r in res.values()), steps_max=max(r[1] for r in res.values()),
+                shape=shape, shape_ms=shape_ms)
 
 
 def main_batch(args):
@@ -196,7 +227,8 @@ def main_batch(args):
         "data": "synthetic (seeded random four-quadrant Riemann states, seed %d)" % BATCH_SEED,
         "config": {"workload": "riemann_batch_%d_uneven_N64-512_rk2_cfl0.5_t0.2" % r["problems"],
                    "precision": "fp64" if prec == cfs.PREC_FP64 else "fp32_bulk",
-                   "total_cells": r["cells"], "positivity_limiter": True, "hll_bounds": "velocity"},
+                   "total_cells": r["cells"], "positivity_limiter": True, "hll_bounds": "velocity",
+                   "tile_shape": shape_report(r)},
         "batched": {"wall_s": round(r["wall"], 3), "solves_per_s": round(r["solves_per_s"], 2),
                     "admission_s": round(r["admit_s"], 3), "failed": r["failed"], "stalled": r["stalled"],
                     "steps_range": [r["steps_min"], r["steps_max"]]},
@@ -235,13 +267,18 @@ def main_serve(args):
     n = args.serve_count
     arena = args.serve_arena_gb << 30
     points = []
+    sample = cfs.serving_requests(SERVE_SEED + 1, 16, rate=1e9, precision=prec)
+    shape, shape_ms = cfs.batch_autotune(sample, rk=2, precision=prec, limiter=True, steps=6, arena_bytes=arena,
+                                         max_problems=args.serve_slots)
+    print("serve autotune: shape %d, ms per step %s" % (shape, shape_ms), file=sys.stderr, flush=True)
     for rate in [float(x) for x in args.serve_rates.split(",")]:
         burst = rate <= 0
         reqs = cfs.serving_requests(SERVE_SEED, n, rate=1e9 if burst else rate, precision=prec)
         if burst:
             for rq in reqs:
                 rq["arrival_s"] = 0.0
-        with cfs.Batch(rk=2, precision=prec, max_problems=args.serve_slots, arena_bytes=arena, limiter=True) as b:
+        with cfs.Batch(rk=2, precision=prec, max_problems=args.serve_slots, arena_bytes=arena, limiter=True,
+                       tile_shape=shape) as b:
             res = b.serve(reqs, steps_per_poll=args.steps_per_poll, keep_states=False)
         lat = np.array([r["latency_s"] for r in res])
         span = max(r["finished_s"] for r in res) - min(rq["arrival_s"] for rq in reqs)
@@ -266,7 +303,8 @@ def main_serve(args):
         "data": "synthetic (seeded request stream, seed %d)" % SERVE_SEED,
         "config": {"workload": "serving_riemann128-1024_cylinder256x128-1024x512_mach1.5-4_t0.1-0.5_rk2_cfl0.5",
                    "arena_gb": args.serve_arena_gb, "slots": args.serve_slots,
-                   "steps_per_poll": args.steps_per_poll},
+                   "steps_per_poll": args.steps_per_poll,
+                   "tile_shape": shape_report(dict(shape=shape, shape_ms=shape_ms))},
         "points": points,
         "sequential_fp64": {"wall_s": round(seq64, 3), "solves_per_s": round(n / seq64, 3), "failed": f64},
         "vs_sequential_fp64": round(cap / (n / seq64), 3),
@@ -292,6 +330,8 @@ def main():
     ap.add_argument("--steps-per-poll", type=int, default=5)
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--batch-fp64", action="store_true", help="batched workload in FP64 instead of FP32-bulk")
+    ap.add_argument("--tile-shape", type=int, default=None, choices=range(cfs.NUM_TILE_SHAPES),
+                    help="stage-kernel tile shape (default: picked by the autotuner)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="N",
                     help="only run N plain copy-kernel launches over one state plane-set (counter calibration)")
     args = ap.parse_args()
@@ -319,22 +359,25 @@ def main():
                                                "bytes_written_per_launch": 32 * NX * NY}}))
         return
     if args.fp32_only:
-        r = run_gpu(cfs.PREC_FP32_BULK, args.steps, args.warmup)
+        r = run_gpu(cfs.PREC_FP32_BULK, args.steps, args.warmup, tile_shape=args.tile_shape)
+        r.pop("rho")
         print(json.dumps({"fp32_bulk_only": r}))
         return
 
-    r64 = run_gpu(cfs.PREC_FP64, args.steps, args.warmup)
+    r64 = run_gpu(cfs.PREC_FP64, args.steps, args.warmup, tile_shape=args.tile_shape)
     if not r64["finite"]:
         print("FP64 state not finite at the end of the timed window", file=sys.stderr)
-    print("fp64: %d steps, %.3f s, %.1f Mcell-updates/s, stage kernel %.4f ms"
-          % (r64["steps"], r64["wall"], r64["mcups"], r64["kernel_ms"]), file=sys.stderr, flush=True)
+    print("fp64: %d steps, %.3f s, %.1f Mcell-updates/s, stage kernel %.4f ms, tile shape %d (autotune ms %s)"
+          % (r64["steps"], r64["wall"], r64["mcups"], r64["kernel_ms"], r64["shape"], r64["shape_ms"]),
+          file=sys.stderr, flush=True)
     r32 = None
     if not args.no_fp32:
-        r32 = run_gpu(cfs.PREC_FP32_BULK, args.steps, args.warmup)
+        r32 = run_gpu(cfs.PREC_FP32_BULK, args.steps, args.warmup, tile_shape=args.tile_shape)
         print("fp32-bulk: %.1f Mcell-updates/s" % r32["mcups"], file=sys.stderr, flush=True)
     rref = None
     if not args.no_reference_scheme:
-        rref = run_gpu(cfs.PREC_FP64, min(args.steps, 500), args.warmup, reference_scheme=True)
+        rref = run_gpu(cfs.PREC_FP64, min(args.steps, 500), args.warmup, reference_scheme=True,
+                       tile_shape=args.tile_shape)
         print("reference scheme (no limiter, subsonic inflow): %.1f Mcell-updates/s, stage kernel %.4f ms"
               % (rref["mcups"], rref["kernel_ms"]), file=sys.stderr, flush=True)
     cpu = None
@@ -365,16 +408,18 @@ def main():
                    "domain": [LX, LY], "cylinder": {"centre": [CX, CY], "radius": RADIUS,
                                                     "lagrangian_points": 400, "ghost_layers": 3},
                    "rk": 3, "cfl": CFL, "precision": "fp64", "stages_per_step": 3,
-                   "ib_interpolation": "bilinear_positive", "finite_at_end": r64["finite"]},
+                   "ib_interpolation": "bilinear_positive", "finite_at_end": r64["finite"],
+                   "tile_shape": shape_report(r64)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
-                     "kernel": "stage_kernel<double, false, true>", "kernel_ms": round(r64["kernel_ms"], 5),
+                     "kernel": kernel_name(cfs.PREC_FP64, True, r64["shape"]),
+                     "kernel_ms": round(r64["kernel_ms"], 5),
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,
     }
     if rref is not None:
         out["reference_scheme"] = {"value": round(rref["mcups"], 2), "unit": "Mcell-updates/s",
-                                   "kernel": "stage_kernel<double, false, false>",
+                                   "kernel": kernel_name(cfs.PREC_FP64, False, rref["shape"]),
                                    "kernel_ms": round(rref["kernel_ms"], 5),
                                    "workload": "same grid and body, inflow rho=u=p=1, no limiter, %d steps"
                                                % rref["steps"]}
@@ -382,7 +427,15 @@ def main():
         out["fp32_bulk"] = {"value": round(r32["mcups"], 2), "unit": "Mcell-updates/s",
                             "ms_per_step": round(r32["ms_per_step"], 4),
                             "kernel_ms": round(r32["kernel_ms"], 5),
-                            "speedup_vs_fp64": round(r32["mcups"] / r64["mcups"], 3)}
+                            "kernel": kernel_name(cfs.PREC_FP32_BULK, True, r32["shape"]),
+                            "tile_shape": shape_report(r32),
+                            "speedup_vs_fp64": round(r32["mcups"] / r64["mcups"], 3),
+                            # both runs: the same initial state and number of CFL steps, each with
+                            # the dt of its own state (FP64 CFL reduction in both modes)
+                            "l1_density_vs_fp64": float(np.abs(r32["rho"] - r64["rho"]).mean()),
+                            "linf_density_vs_fp64": float(np.abs(r32["rho"] - r64["rho"]).max()),
+                            "compared_after_steps": r64["total_steps"],
+                            "time_fp64": r64["t_end"], "time_fp32_bulk": r32["t_end"]}
     print(json.dumps(out))
 
 

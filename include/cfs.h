@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define CFS_ABI_VERSION 2
+#define CFS_ABI_VERSION 3
 
 typedef enum cfs_status {
     CFS_OK = 0,
@@ -218,6 +218,27 @@ int32_t cfs_set_kernel_timing(cfs_solver* s, int32_t on);
  * with kernel timing on (0 if none). */
 int32_t cfs_last_kernel_ms(const cfs_solver* s, double* ms_per_stage);
 
+/* ---- tile shapes and the autotuner ---------------------------------------- */
+/* The stage kernels come in CFS_NUM_TILE_SHAPES tile shapes: a workgroup of
+ * `threads` threads updates ti x tj cells (ti along x, tj along y).  Results do
+ * not depend on the shape (every face and cell is computed the same way);
+ * speed does, with the grid, the precision and the kernel variant.  Shape 0
+ * (15 x 16, 256 threads) is the default and the only one of the viscous
+ * variant.  No reference counterpart: the reference has no tiling. */
+#define CFS_NUM_TILE_SHAPES 3
+int32_t cfs_tile_shape_info(int32_t shape, int32_t* ti, int32_t* tj, int32_t* threads);
+/* Selects the tile shape of later stages (CFS_E_UNSUPPORTED for a viscous
+ * configuration and shape != 0).  Waits for queued work. */
+int32_t cfs_set_tile_shape(cfs_solver* s, int32_t shape);
+int32_t cfs_get_tile_shape(const cfs_solver* s, int32_t* shape);
+/* Autotuner: times `reps` launches of the stage kernel of every tile shape on
+ * the handle's current state and configuration (sub-stage 0 with dt = 0
+ * written to a spare plane-set: the state is not changed), selects the
+ * fastest and reports it in *best (may be NULL) and the mean milliseconds per
+ * launch of each shape in ms[CFS_NUM_TILE_SHAPES] (may be NULL; -1 for shapes
+ * the configuration has not).  At a step boundary only (CFS_E_STATE). */
+int32_t cfs_autotune(cfs_solver* s, int32_t reps, int32_t* best, double* ms);
+
 /* ---- batched solves ------------------------------------------------------- */
 /* Many independent problems of uneven sizes advanced together: each call of
  * cfs_batch_step runs one persistent stage kernel per RK sub-stage over the
@@ -239,7 +260,8 @@ typedef struct cfs_batch_config {
     int32_t workgroups;      /* persistent grid; 0: CUs x resident workgroups per CU */
     int64_t arena_bytes;     /* device memory for problem state and tables */
     int32_t limiter;         /* 1: kernel with the positivity limiter (problems may use positivity = 2) */
-    int32_t reserved;        /* 0 */
+    int32_t tile_shape;      /* stage-kernel tile shape, 0 .. CFS_NUM_TILE_SHAPES-1
+                                (cfs_tile_shape_info; cfs_batch_autotune picks one) */
 } cfs_batch_config;
 
 /* Immersed-boundary tables of one problem, as cfs_set_immersed takes them. */
@@ -291,6 +313,18 @@ int32_t cfs_batch_retire(cfs_batch* b, int32_t slot, double* U);
 int32_t cfs_batch_free_bytes(const cfs_batch* b, int64_t* bytes);
 /* Arena size and slot count the batch was created with. */
 int32_t cfs_batch_capacity(const cfs_batch* b, int64_t* arena_bytes, int32_t* max_problems);
+
+struct cfs_request;
+/* Autotuner of the batched kernel: for every tile shape, creates a batch of
+ * configuration *bc with that shape (bc->tile_shape is ignored), admits the n
+ * sample problems of `sample` (cfg, U, ib and cfl are read; each runs without
+ * an end time), runs 2 warm-up steps and times `steps` RK steps of the whole
+ * batch with HIP events.  *best receives the fastest shape, ms_per_step
+ * [CFS_NUM_TILE_SHAPES] the milliseconds per step of each (may be NULL).
+ * CFS_E_FULL if the sample does not fit the arena.  No reference
+ * counterpart. */
+int32_t cfs_batch_autotune(const cfs_batch_config* bc, const struct cfs_request* sample, int32_t n, int32_t steps,
+                           int32_t* best, double* ms_per_step);
 
 /* ---- serving loop --------------------------------------------------------- */
 /* A request: one solve to t_end, arriving at arrival_s seconds after the

@@ -96,3 +96,20 @@ def test_riemann_batch_problems_are_seeded():
     sizes = [c.nx for c, _ in cfs.riemann_batch_problems(1, count=256)]
     assert min(sizes) >= 64 and max(sizes) <= 512 and all(n % 32 == 0 for n in sizes)
     assert len(set(sizes)) > 10
+
+
+def test_tile_shapes_without_a_gpu():
+    lib = cfs.library()
+    shapes = [cfs.tile_shape_info(k) for k in range(cfs.NUM_TILE_SHAPES)]
+    assert shapes[0] == (15, 16, 256)
+    for ti, tj, nt in shapes:
+        # one cell per thread, the faces in two rounds of the workgroup
+        assert ti * tj <= nt and (ti + 1) * tj + ti * (tj + 1) <= 2 * nt and nt % 64 == 0
+    assert "#define CFS_NUM_TILE_SHAPES %d" % cfs.NUM_TILE_SHAPES in open(cfs.HEADER_PATH).read()
+    assert lib.cfs_tile_shape_info(-1, None, None, None) == cfs.CFS_E_ARG
+    assert lib.cfs_set_tile_shape(None, 0) == cfs.CFS_E_ARG
+    assert lib.cfs_autotune(None, 5, None, None) == cfs.CFS_E_ARG
+    h = ctypes.c_void_p()
+    bc = cfs.BatchConfig(3, cfs.PREC_FP64, 4, 0, 1 << 20, 0, cfs.NUM_TILE_SHAPES)
+    assert lib.cfs_batch_create(ctypes.byref(bc), None, ctypes.byref(h)) == cfs.CFS_E_ARG and not h.value
+    assert lib.cfs_batch_autotune(ctypes.byref(bc), None, 0, 5, None, None) == cfs.CFS_E_ARG
