@@ -4,6 +4,9 @@
 // [4][nx][ny] (x-major, j contiguous), used as RK ping-pong buffers; an
 // optional uint8 cell-type plane [nx][ny]; immersed-boundary tables.
 //
+// LDS holds the staged state and the face fluxes as planes per variable
+// (structure of arrays), so the lanes of a wave read consecutive doubles.
+//
 // Stage kernel (one launch per RK stage): a 256-thread workgroup owns a
 // TI x TJ cell tile (15 x 16 with 256 threads by default, larger shapes
 // below).  It stages the tile plus a 3-cell halo
@@ -230,16 +233,17 @@ __device__ __forceinline__ R weno5(R v0, R v1, R v2, R v3, R v4) {
     return (w0 * q0 + w1 * q1) + w2 * q2;
 }
 
-// Face flux from the 6-cell stencil in the face-normal frame.
-// st[k][0..3] = (rho, normal momentum, tangential momentum, E) of stencil
-// cell k (offsets -3..+2).  Left state: WENO over cells 0..4; right state:
+// Face flux from the 6-cell stencil in the face-normal frame.  S holds the
+// staged state as four planes of ns cells (rho, rho u, rho v, E); cell[k] is
+// the LDS cell of stencil offset k - 3, read as (rho, normal momentum,
+// tangential momentum, E).  Left state: WENO over cells 0..4; right state:
 // over cells 5..1 (Weno5th.cpp:57-81).  Pressure (EulerPhase.cpp:130-133),
 // Euler flux (EulerMod.cpp:33-54) and HLL (RiemmanSolverHLL.cpp:5-75) follow.
 // The x and y forms of the reference map onto this one form because IEEE
 // addition and multiplication are commutative (mx*my == my*mx bit for bit).
 // out = (mass, normal momentum, tangential momentum, energy) flux.
 template <typename R, bool FAST>
-__device__ __forceinline__ void face_flux(const R* __restrict__ S, const int cell[6], int nrm, int tng,
+__device__ __forceinline__ void face_flux(const R* __restrict__ S, int ns, const int cell[6], int nrm, int tng,
                                           R g, int positivity, int hll_bounds, R out[4]) {
     using C = K<R>;
     R L[4], Rr[4], cl[4], cr[4];
@@ -248,7 +252,7 @@ __device__ __forceinline__ void face_flux(const R* __restrict__ S, const int cel
     for (int q = 0; q < 4; ++q) {
         R v[6];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) v[k] = S[cell[k] + comp[q]];
+        for (int k = 0; k < 6; ++k) v[k] = S[comp[q] * ns + cell[k]];
         L[q] = weno5<R, FAST>(v[0], v[1], v[2], v[3], v[4]);
         Rr[q] = weno5<R, FAST>(v[5], v[4], v[3], v[2], v[1]);
         cl[q] = v[2];
@@ -519,14 +523,14 @@ __device__ __forceinline__ void euler_flux_n(R r, R n, R t, R E, R gm1, R F[4]) 
 // cL, cR: LDS cells left and right of the face; dn, dt: LDS offsets of the
 // neighbours along the normal and the tangent; ln, lt: dt/h along them
 template <typename R, typename Sh>
-__device__ __forceinline__ void limit_face(const R* __restrict__ S, const LimitLDS<R, Sh>& Lm, int cL, int cR, int nrm,
+__device__ __forceinline__ void limit_face(const R* __restrict__ S, int ns, const LimitLDS<R, Sh>& Lm, int cL, int cR, int nrm,
                                            int tng, int dn, int dtg, R ln, R lt, R g, R fl[4]) {
     using C = K<R>;
     const R* sn = (nrm == 1) ? Lm.sx : Lm.sy;
     const R* st = (nrm == 1) ? Lm.sy : Lm.sx;
     const R gm1 = g - R(1);
-    const R UL[4] = {S[cL * 4], S[cL * 4 + nrm], S[cL * 4 + tng], S[cL * 4 + 3]};
-    const R UR[4] = {S[cR * 4], S[cR * 4 + nrm], S[cR * 4 + tng], S[cR * 4 + 3]};
+    const R UL[4] = {S[cL], S[nrm * ns + cL], S[tng * ns + cL], S[3 * ns + cL]};
+    const R UR[4] = {S[cR], S[nrm * ns + cR], S[tng * ns + cR], S[3 * ns + cR]};
     R FL[4], FR[4];
     euler_flux_n<R>(UL[0], UL[1], UL[2], UL[3], gm1, FL);
     euler_flux_n<R>(UR[0], UR[1], UR[2], UR[3], gm1, FR);
@@ -643,6 +647,7 @@ template <typename R, bool VISC, bool LIMIT, bool FAST, typename Sh>
 __device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, double dt, const R* S, R* F,
                                            const ViscLDS<R, Sh>* V, const LimitLDS<R, Sh>* Lm) {
     constexpr int TJ = Sh::TJ, SC = Sh::SC, NXF = Sh::NXF, NYF = Sh::NYF, BLOCK = Sh::NT;
+    constexpr int NS = Sh::SR * Sh::SC, NF = NXF + NYF;   // LDS plane lengths (state, fluxes)
     const int tid = threadIdx.x;
     const bool px = a.bc[0] == BC_PERIODIC, py = a.bc[2] == BC_PERIODIC;
     const R g = (R)a.gamma;
@@ -660,7 +665,7 @@ __device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, d
                 // stencil is [G,G,G,c0,c0,c0] at f = 0 and [c,c,c,G,G,G] at f = n
                 if (!px && f == 0 && o > 0) o = 0;
                 if (!px && f == a.nx && o < -1) o = -1;
-                cell[k] = ((fr + o + HALO) * SC + jl + HALO) * 4;
+                cell[k] = (fr + o + HALO) * SC + jl + HALO;
             }
             nrm = 1; tng = 2;
         } else {
@@ -672,23 +677,23 @@ __device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, d
                 int o = k - 3;
                 if (!py && gf == 0 && o > 0) o = 0;
                 if (!py && gf == a.ny && o < -1) o = -1;
-                cell[k] = ((il + HALO) * SC + gl + o + HALO) * 4;
+                cell[k] = (il + HALO) * SC + gl + o + HALO;
             }
             nrm = 2; tng = 1;
         }
         R fl[4];
-        face_flux<R, FAST>(S, cell, nrm, tng, g, a.positivity, a.hll_bounds, fl);
+        face_flux<R, FAST>(S, NS, cell, nrm, tng, g, a.positivity, a.hll_bounds, fl);
         if constexpr (LIMIT) {
             if (a.positivity == 2) {
                 const bool xf = s < NXF;
-                limit_face<R, Sh>(S, *Lm, cell[2] >> 2, cell[3] >> 2, nrm, tng, xf ? SC : 1, xf ? 1 : SC,
+                limit_face<R, Sh>(S, NS, *Lm, cell[2], cell[3], nrm, tng, xf ? SC : 1, xf ? 1 : SC,
                               xf ? lam_x : lam_y, xf ? lam_y : lam_x, g, fl);
             }
         }
         if constexpr (VISC) {   // net flux = inviscid - viscous (mass: minus zero)
             int cs[6];
 #pragma unroll
-            for (int k = 0; k < 6; ++k) cs[k] = cell[k] >> 2;
+            for (int k = 0; k < 6; ++k) cs[k] = cell[k];
             const bool xface = s < NXF;
             R fv[3];
             viscous_flux<R, Sh>(*V, cs, xface, xface ? (R)a.dx : (R)a.dy, (R)a.mu, (R)a.kappa, fv);
@@ -696,10 +701,10 @@ __device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, d
             fl[2] = fl[2] - fv[1];
             fl[3] = fl[3] - fv[2];
         }
-        F[s * 4 + 0] = fl[0];
-        F[s * 4 + nrm] = fl[1];
-        F[s * 4 + tng] = fl[2];
-        F[s * 4 + 3] = fl[3];
+        F[s] = fl[0];
+        F[nrm * NF + s] = fl[1];
+        F[tng * NF + s] = fl[2];
+        F[3 * NF + s] = fl[3];
     }
 }
 
@@ -707,6 +712,7 @@ template <typename R, bool VISC, bool LIMIT, typename Sh>
 __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R* F, double* wmax,
                                            ViscLDS<R, Sh>* V, LimitLDS<R, Sh>* Lm) {
     constexpr int TI = Sh::TI, TJ = Sh::TJ, SR = Sh::SR, SC = Sh::SC, NXF = Sh::NXF, BLOCK = Sh::NT;
+    constexpr int NS = SR * SC, NF = NXF + Sh::NYF;   // LDS plane lengths (state, fluxes)
     const int tid = threadIdx.x;
     const int ti = tile / a.tiles_j;
     const int tj = tile - ti * a.tiles_j;
@@ -776,7 +782,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            S[s * 4 + q] = (R)v[q];
+            S[q * NS + s] = (R)v[q];
             if (sizeof(R) == 8) unsafe |= !fast_div_safe(v[q]);
         }
     }
@@ -790,7 +796,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
             // |u| + c and |v| + c of every staged cell
             const R g = (R)a.gamma, gm1 = g - R(1);
             for (int s = tid; s < SR * SC; s += BLOCK) {
-                const R rho = S[s * 4], mx = S[s * 4 + 1], my = S[s * 4 + 2], E = S[s * 4 + 3];
+                const R rho = S[s], mx = S[NS + s], my = S[2 * NS + s], E = S[3 * NS + s];
                 const R ir = K<R>::div(R(1), rho);
                 const R ux = mx * ir, uy = my * ir;
                 const R p = gm1 * (E - K<R>::half * (mx * ux + my * uy));
@@ -806,7 +812,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
         // primitive variables of every staged cell (EulerPhase.cpp:130-137 forms)
         const R g = (R)a.gamma, gm1 = g - R(1), gma2 = g * (R)a.ma2;
         for (int s = tid; s < SR * SC; s += BLOCK) {
-            const R rho = S[s * 4], mx = S[s * 4 + 1], my = S[s * 4 + 2], E = S[s * 4 + 3];
+            const R rho = S[s], mx = S[NS + s], my = S[2 * NS + s], E = S[3 * NS + s];
             const R p = gm1 * (E - K<R>::div(K<R>::half * (mx * mx + my * my), rho));
             V->pu[s] = K<R>::div(mx, rho);
             V->pv[s] = K<R>::div(my, rho);
@@ -864,14 +870,14 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 // the FP64 tile holds the exact state; FP32-bulk re-reads it
-                const double u = (sizeof(R) == 8) ? (double)S[((il + HALO) * SC + jl + HALO) * 4 + q]
+                const double u = (sizeof(R) == 8) ? (double)S[q * NS + (il + HALO) * SC + jl + HALO]
                                                   : a.in[q * a.plane + c];
                 res[q] = u;
                 if (ty == 0) {
-                    const R fl = F[(il * TJ + jl) * 4 + q];
-                    const R fr = F[((il + 1) * TJ + jl) * 4 + q];
-                    const R gb = F[(NXF + il * (TJ + 1) + jl) * 4 + q];
-                    const R gt = F[(NXF + il * (TJ + 1) + jl + 1) * 4 + q];
+                    const R fl = F[q * NF + il * TJ + jl];
+                    const R fr = F[q * NF + (il + 1) * TJ + jl];
+                    const R gb = F[q * NF + NXF + il * (TJ + 1) + jl];
+                    const R gt = F[q * NF + NXF + il * (TJ + 1) + jl + 1];
                     // x/d == x*(1/d) bit for bit when d is a power of two
                     const R tx = a.dx_pow2 ? (-(fr - fl)) * idx : K<R>::div(-(fr - fl), rdx);
                     const R tyy = a.dy_pow2 ? (gt - gb) * idy : K<R>::div(gt - gb, rdy);
