@@ -228,7 +228,9 @@ __device__ __forceinline__ R weno5(R v0, R v1, R v2, R v3, R v4) {
     const R rs = D::rcp(sum);
     const R w0 = D::div_r(a0, sum, rs), w1 = D::div_r(a1, sum, rs), w2 = D::div_r(a2, sum, rs);
     const R q0 = (D::d3(v0) - D::d6(R(7) * v1)) + D::d6(R(11) * v2);
-    const R q1 = (D::d6(-v1) + D::d6(R(5) * v2)) + D::d3(v3);
+    // d6(-x) == -d6(x) bit for bit (round-to-nearest is odd-symmetric): the
+    // right state's d6(v1) is then the left state's d6(v4) and vice versa
+    const R q1 = (-D::d6(v1) + D::d6(R(5) * v2)) + D::d3(v3);
     const R q2 = (D::d3(v2) + D::d6(R(5) * v3)) - D::d6(v4);
     return (w0 * q0 + w1 * q1) + w2 * q2;
 }
