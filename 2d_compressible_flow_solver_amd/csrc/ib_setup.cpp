@@ -189,14 +189,31 @@ extern "C" int32_t cfs_circle_setup(const cfs_config* cfg, double cx, double cy,
     auto T = [&](int i, int j) -> uint8_t& { return cell_type[(size_t)i * ny + j]; };
 
     const CircleGeometry circle(cx, cy, radius, n_lagrangian);
-    for (int i = 0; i < nx; ++i)
-        for (int j = 0; j < ny; ++j) T(i, j) = circle.solid(centre(i, j)) ? CFS_CELL_SOLID : CFS_CELL_FLUID;
+    // Only cells of the body's bounding box, widened by 3 cells, can be solid:
+    // a cell centre farther than r + 2 dx from the centre along x or y is at
+    // a distance > r whatever the rounding.  Classification, ghost marking and
+    // the ghost list scan that box (plus one cell for the marking walks, which
+    // start at fluid cells next to the body) instead of the whole grid.
+    std::fill(cell_type, cell_type + (size_t)nx * ny, (uint8_t)CFS_CELL_FLUID);
+    auto clampi = [](double v, int lo, int hi) {
+        if (!(v > lo)) return lo;   // also NaN
+        if (!(v < hi)) return hi;
+        return (int)v;
+    };
+    const int i0 = clampi(std::floor((cx - radius - cfg->x0) / dx) - 3, 0, nx - 1);
+    const int i1 = clampi(std::ceil((cx + radius - cfg->x0) / dx) + 3, 0, nx - 1);
+    const int j0 = clampi(std::floor((cy - radius - cfg->y0) / dy) - 3, 0, ny - 1);
+    const int j1 = clampi(std::ceil((cy + radius - cfg->y0) / dy) + 3, 0, ny - 1);
+    for (int i = i0; i <= i1; ++i)
+        for (int j = j0; j <= j1; ++j) T(i, j) = circle.solid(centre(i, j)) ? CFS_CELL_SOLID : CFS_CELL_FLUID;
 
     // from each fluid cell, walk into the body along the 4 grid directions and
     // mark up to n_layers consecutive non-fluid cells as immersed ghosts
     static const int step[4][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}};
-    for (int i = 0; i < nx; ++i)
-        for (int j = 0; j < ny; ++j) {
+    const int wi0 = std::max(i0 - 1, 0), wi1 = std::min(i1 + 1, nx - 1);
+    const int wj0 = std::max(j0 - 1, 0), wj1 = std::min(j1 + 1, ny - 1);
+    for (int i = wi0; i <= wi1; ++i)
+        for (int j = wj0; j <= wj1; ++j) {
             if (T(i, j) != CFS_CELL_FLUID) continue;
             for (const auto& d : step) {
                 for (int k = 1; k <= n_layers; ++k) {
@@ -211,8 +228,8 @@ extern "C" int32_t cfs_circle_setup(const cfs_config* cfg, double cx, double cy,
     int count = 0;
     bool short_window = false;
     std::vector<std::pair<double, int>> cand;
-    for (int i = 0; i < nx; ++i)
-        for (int j = 0; j < ny; ++j) {
+    for (int i = i0; i <= i1; ++i)
+        for (int j = j0; j <= j1; ++j) {
             if (T(i, j) != CFS_CELL_IMMERSED_GHOST) continue;
             const int k = count++;
             if (k >= cap) continue;

@@ -30,6 +30,7 @@ the same workload, all host threads (up to 16), a bounded number of steps.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
        python bench.py --workload batch | serve    (batched / serving measurements)
+       python bench.py --workload kh               (viscous shear layer 4096^2 to t = 1, FP64 and FP32-bulk)
 """
 import argparse
 import importlib
@@ -239,6 +240,59 @@ def main_batch(args):
     print(json.dumps(out))
 
 
+def run_kh(precision, n, t_end, chunk=1000):
+    """The viscous shear layer from its seeded initial state to t_end with CFL
+    steps; progress every chunk steps.  Returns timings and the density field."""
+    cfg = cfs.config_kelvin_helmholtz(n, precision=precision)
+    s = cfs.Solver(cfg)
+    s.set_state(cfs.kelvin_helmholtz(cfg))
+    t, steps, status = 0.0, 0, "ok"
+    t0 = time.perf_counter()
+    while t < t_end:
+        try:
+            t, done = s.advance(chunk, dt=0.0, cfl=CFL, t_end=t_end, t=t)
+        except cfs.CfsError as e:
+            status = str(e)
+            break
+        steps += done
+        print("kh %s: %d steps, t = %.4f, %.1f s" % ("fp64" if precision == cfs.PREC_FP64 else "fp32-bulk", steps,
+                                                    t, time.perf_counter() - t0), file=sys.stderr, flush=True)
+        if done == 0:
+            break
+    wall = time.perf_counter() - t0
+    rho = s.get_state()[0]
+    s.set_kernel_timing(True)
+    s.advance(5, dt=0.0, cfl=CFL, t_end=0.0, t=t)
+    kms = s.last_kernel_ms()
+    s.close()
+    return dict(steps=steps, t=t, wall=wall, mcups=n * n * 3 * steps / wall / 1e6, kernel_ms=kms, rho=rho,
+                status=status, finite=bool(np.all(np.isfinite(rho))))
+
+
+def main_kh(args):
+    n, t_end = args.kh_n, args.kh_t_end
+    r64 = run_kh(cfs.PREC_FP64, n, t_end)
+    r32 = run_kh(cfs.PREC_FP32_BULK, n, t_end)
+    l1 = float(np.abs(r32["rho"] - r64["rho"]).mean())
+    out = {
+        "metric": "Mcell-updates/s per RK stage (viscous shear layer)",
+        "value": round(r64["mcups"], 2),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f64",
+        "data": "synthetic (seeded shear layer, noise seed %d)" % cfs.KH_SEED,
+        "config": {"workload": "kelvin_helmholtz_%dx%d_viscous_re1e5_pr0.72_rk3_cfl0.5_t%g" % (n, n, t_end),
+                   "periodic": True},
+        "fp64": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in r64.items() if k != "rho"},
+        "fp32_bulk": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in r32.items() if k != "rho"},
+        "fp32_bulk_speedup": round(r32["mcups"] / r64["mcups"], 3),
+        "l1_density_fp32_bulk_vs_fp64": l1,
+        "linf_density_fp32_bulk_vs_fp64": float(np.abs(r32["rho"] - r64["rho"]).max()),
+    }
+    print(json.dumps(out))
+
+
 SERVE_SEED = 7
 
 
@@ -321,8 +375,10 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-bulk leg")
     ap.add_argument("--no-reference-scheme", action="store_true", help="skip the no-limiter leg")
     ap.add_argument("--fp32-only", action="store_true", help="time the FP32-bulk leg only (profiling)")
-    ap.add_argument("--workload", choices=["cylinder", "batch", "serve"], default="cylinder",
+    ap.add_argument("--workload", choices=["cylinder", "batch", "serve", "kh"], default="cylinder",
                     help="cylinder: the headline line; batch / serve: the batched-solves and serving measurements")
+    ap.add_argument("--kh-n", type=int, default=4096, help="shear-layer grid side (--workload kh)")
+    ap.add_argument("--kh-t-end", type=float, default=1.0, help="shear-layer end time (--workload kh)")
     ap.add_argument("--serve-count", type=int, default=48)
     ap.add_argument("--serve-rates", default="1,2,4,0", help="offered request rates per second (0: burst)")
     ap.add_argument("--serve-arena-gb", type=int, default=4)
@@ -348,6 +404,9 @@ def main():
         return
     if args.workload == "serve":
         main_serve(args)
+        return
+    if args.workload == "kh":
+        main_kh(args)
         return
     if args.copy_calibration:
         cfg, ib, U = setup(cfs.PREC_FP64)

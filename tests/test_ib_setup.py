@@ -24,6 +24,11 @@ CASES = [
     (128, 64, 4.0, 2.0, 1.0, 1.0, 0.25),
     (200, 200, 2.0, 2.0, 0.5, 1.0, 0.1),
     (333, 170, 4.0, 2.0, 1.3, 0.9, 0.137),
+    # bodies cut by the domain edge: the set-up scans only the body's
+    # bounding box, which is then clipped to the grid
+    (160, 80, 4.0, 2.0, 0.1, 1.0, 0.25),
+    (96, 96, 1.0, 1.0, 0.95, 0.95, 0.2),
+    (2048, 1024, 4.0, 2.0, 1.0, 1.0, 0.125),
 ]
 
 
