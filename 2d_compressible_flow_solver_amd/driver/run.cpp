@@ -89,7 +89,10 @@ int Run::solver() {
         t_ = opt_.dt * stages_ / cfg_.rk;
         return CFS_OK;
     }
-    // CFL steps on the device to t_end, snapshots every output_every steps
+    // CFL steps on the device to t_end, snapshots every output_every steps;
+    // the stage kernel's tile shape is tuned on the initial state first (the
+    // results do not depend on it)
+    if ((rc = cfs_autotune(h_, 10, nullptr, nullptr))) return rc;
     const int chunk = opt_.output_every > 0 ? opt_.output_every : 1000000;
     int k = 0;
     while (opt_.t_end > 0.0 && t_ < opt_.t_end) {

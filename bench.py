@@ -26,11 +26,14 @@ roofline: the stage kernel, algorithmic bytes per launch = cells * 97 B
 (RK3 mean 96 B/cell-stage of FP64 state planes + 1 B cell-type mask) over its
 mean launch duration measured with HIP events on the library's stream.
 cpu_baseline: the strict-IEEE CPU restatement of the reference (oracle/) on
-the same workload, all host threads (up to 16), a bounded number of steps.
+the same workload in the reference's MPI form (2D block decomposition, one
+rank per host thread up to 16, 3-cell ghost frames exchanged every stage), a
+bounded number of steps.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
        python bench.py --workload batch | serve    (batched / serving measurements)
        python bench.py --workload kh               (viscous shear layer 4096^2 to t = 1, FP64 and FP32-bulk)
+       python bench.py --workload riemann          (Lax-Liu 400^2 to t = 0.3, GPU vs CPU restatement, bitwise)
 """
 import argparse
 import importlib
@@ -129,6 +132,7 @@ def run_cpu_baseline(budget_s=12.0):
     cfg, ib, U = setup(cfs.PREC_FP64)
     nthreads = max(1, min(16, os.cpu_count() or 1))
     o = O.Oracle(cfg, nthreads=nthreads)
+    dims = o.set_ranks(nthreads)   # the reference's MPI decomposition, one rank per host thread
     o.set_ib_table(ib)
     o.set_state(U)
     steps = 0
@@ -144,7 +148,9 @@ def run_cpu_baseline(budget_s=12.0):
     return dict(value=NX * NY * 3 * steps / wall / 1e6, unit="Mcell-updates/s", cores=nthreads,
                 kind="port",
                 sample="%d RK3 steps (%d stages incl. per-step CFL) of the same 2048x1024 cylinder "
-                       "workload, %.1f s, strict-IEEE CPU restatement of the reference" % (steps, 3 * steps, wall))
+                       "workload, %.1f s, strict-IEEE CPU restatement of the reference in its MPI form: "
+                       "%dx%d blocks, one rank per thread, 3-cell ghost frames exchanged every stage"
+                       % (steps, 3 * steps, wall, dims[0], dims[1]))
 
 
 BATCH_SEED = 2024
@@ -236,6 +242,63 @@ def main_batch(args):
         "one_after_another": {"wall_s": round(r["seq_wall"], 3), "value": round(r["seq_mcups"], 2),
                               "solves_per_s": round(r["seq_solves_per_s"], 2)},
         "speedup_vs_one_after_another": round(r["seq_wall"] / r["wall"], 3),
+    }
+    print(json.dumps(out))
+
+
+def main_riemann(args):
+    """The reference's own CPU-runnable case end to end: the Lax-Liu
+    four-quadrant problem on 400 x 400, WENO5 + HLL + TVD-RK3, CFL 0.5, FP64,
+    transmissive boundaries, to t = 0.3; on the GPU (CFL steps on the device),
+    then the CPU restatement on all host threads with the same time steps
+    (dt = 0.5 / s_max of its own state, bit-identical to the device's), and the
+    final states compared bit for bit."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as O
+    cfg = cfs.config_riemann(400, rk=3)
+    U = cfs.riemann_quadrants(cfg)
+    s = cfs.Solver(cfg)
+    s.set_state(U)
+    s.advance(2, dt=1e-9)   # warm-up: module load, first launches (not the measured solve)
+    s.set_state(U)
+    t0 = time.perf_counter()
+    t, steps = s.advance(100000, dt=0.0, cfl=CFL, t_end=0.3)
+    gpu_wall = time.perf_counter() - t0
+    G = s.get_state()
+    s.close()
+    nthreads = max(1, min(16, os.cpu_count() or 1))
+    o = O.Oracle(cfg, nthreads=nthreads)
+    o.set_ranks(nthreads)
+    o.set_state(U)
+    tc, n = 0.0, 0
+    t0 = time.perf_counter()
+    while tc < 0.3:
+        dt = CFL / o.cfl_smax()
+        rem = 0.3 - tc                # the device's end-time clip
+        if not (dt < rem):
+            dt = rem
+        o.step(dt)
+        tc += dt
+        n += 1
+    cpu_wall = time.perf_counter() - t0
+    C = o.get_state()
+    o.close()
+    cells = cfg.nx * cfg.ny
+    out = {
+        "metric": "Mcell-updates/s per RK stage (Lax-Liu 400x400 to t = 0.3)",
+        "value": round(cells * 3 * steps / gpu_wall / 1e6, 2),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f64",
+        "data": "Lax-Liu configuration 3 quadrant states (no randomness)",
+        "config": {"workload": "riemann_laxliu3_400x400_rk3_cfl0.5_t0.3_fp64"},
+        "gpu": {"steps": steps, "t": t, "wall_s": round(gpu_wall, 4)},
+        "cpu_baseline": {"value": round(cells * 3 * n / cpu_wall / 1e6, 3), "unit": "Mcell-updates/s",
+                         "cores": nthreads, "kind": "port", "steps": n, "wall_s": round(cpu_wall, 3),
+                         "sample": "the whole solve to t = 0.3"},
+        "bit_identical_at_t_end": bool(np.array_equal(G.view(np.uint64), C.view(np.uint64)) and n == steps),
+        "max_abs_diff": float(np.abs(G - C).max()),
     }
     print(json.dumps(out))
 
@@ -375,7 +438,7 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-bulk leg")
     ap.add_argument("--no-reference-scheme", action="store_true", help="skip the no-limiter leg")
     ap.add_argument("--fp32-only", action="store_true", help="time the FP32-bulk leg only (profiling)")
-    ap.add_argument("--workload", choices=["cylinder", "batch", "serve", "kh"], default="cylinder",
+    ap.add_argument("--workload", choices=["cylinder", "batch", "serve", "kh", "riemann"], default="cylinder",
                     help="cylinder: the headline line; batch / serve: the batched-solves and serving measurements")
     ap.add_argument("--kh-n", type=int, default=4096, help="shear-layer grid side (--workload kh)")
     ap.add_argument("--kh-t-end", type=float, default=1.0, help="shear-layer end time (--workload kh)")
@@ -407,6 +470,9 @@ def main():
         return
     if args.workload == "kh":
         main_kh(args)
+        return
+    if args.workload == "riemann":
+        main_riemann(args)
         return
     if args.copy_calibration:
         cfg, ib, U = setup(cfs.PREC_FP64)

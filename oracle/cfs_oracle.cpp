@@ -23,6 +23,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
+#include <mutex>
 #include <cstdint>
 #include <cstring>
 #include <deque>
@@ -224,6 +226,7 @@ struct Oracle {
     int ib_interp = CFS_IB_REFERENCE;
     int counter = 1;
     int nthreads = 1;
+    int ranks = 1, dims[2] = {1, 1};  // multi-rank mode (stage_ranks)
 
     size_t at(int q, int i, int j) const { return (size_t)q * nx * ny + (size_t)i * ny + j; }
     bool periodic_x() const { return cfg.bc[0] == CFS_BC_PERIODIC; }
@@ -241,6 +244,20 @@ void parallel_for(int n, int nthreads, F f) {
         th.emplace_back([=]() { f(a, b); });
     }
     for (auto& t : th) t.join();
+}
+
+// Where the face phase reads interior cell states: the global state, or -- in
+// the multi-rank mode -- the reading rank's own block plus its exchanged
+// 3-cell ghost frame (set per worker thread).
+struct View {
+    const double* U;
+    size_t qs;
+    int ld, i0, j0;
+    double at(int q, int i, int j) const { return U[q * qs + (size_t)(i - i0) * ld + (j - j0)]; }
+};
+thread_local const View* tl_view = nullptr;
+inline double ucell(const Oracle& o, int q, int i, int j) {
+    return tl_view ? tl_view->at(q, i, j) : o.U[o.at(q, i, j)];
 }
 
 // CellInterface::collectCells (CellInterface.cpp:135-195) restated on
@@ -264,22 +281,34 @@ void collect_stencil(int f, int N, bool periodic, int out[6]) {
     for (int k = 0; k < 6; ++k) out[k] = cells[k];
 }
 
+// The multi-rank mode's stencil: periodic indices stay unwrapped (the frame
+// holds the wrapped copies there); otherwise the reference's face walk.
+void collect_stencil_rank(int f, int N, bool periodic, int out[6]) {
+    if (periodic) {
+        for (int k = 0; k < 6; ++k) out[k] = f - 3 + k;
+        return;
+    }
+    collect_stencil(f, N, false, out);
+}
+
 Phase cell_phase_x(const Oracle& o, int i, int j) {
     // i in [-1, nx]: -1 and nx are the left/right ghost cells of row j
     Phase s{};
     double v[4];
-    if (i == -1) for (int q = 0; q < 4; ++q) v[q] = o.ghost[0][(size_t)q * o.ny + j];
-    else if (i == o.nx) for (int q = 0; q < 4; ++q) v[q] = o.ghost[1][(size_t)q * o.ny + j];
-    else for (int q = 0; q < 4; ++q) v[q] = o.U[o.at(q, i, j)];
+    // (periodic axes never reach a ghost: their stencils are wrapped, or, in
+    // the multi-rank mode, read from the rank's frame at unwrapped indices)
+    if (i == -1 && !o.periodic_x()) for (int q = 0; q < 4; ++q) v[q] = o.ghost[0][(size_t)q * o.ny + j];
+    else if (i == o.nx && !o.periodic_x()) for (int q = 0; q < 4; ++q) v[q] = o.ghost[1][(size_t)q * o.ny + j];
+    else for (int q = 0; q < 4; ++q) v[q] = ucell(o, q, i, j);
     s.rho = v[0]; s.mu = v[1]; s.mv = v[2]; s.E = v[3];
     return s;
 }
 Phase cell_phase_y(const Oracle& o, int i, int j) {
     Phase s{};
     double v[4];
-    if (j == -1) for (int q = 0; q < 4; ++q) v[q] = o.ghost[2][(size_t)q * o.nx + i];
-    else if (j == o.ny) for (int q = 0; q < 4; ++q) v[q] = o.ghost[3][(size_t)q * o.nx + i];
-    else for (int q = 0; q < 4; ++q) v[q] = o.U[o.at(q, i, j)];
+    if (j == -1 && !o.periodic_y()) for (int q = 0; q < 4; ++q) v[q] = o.ghost[2][(size_t)q * o.nx + i];
+    else if (j == o.ny && !o.periodic_y()) for (int q = 0; q < 4; ++q) v[q] = o.ghost[3][(size_t)q * o.nx + i];
+    else for (int q = 0; q < 4; ++q) v[q] = ucell(o, q, i, j);
     s.rho = v[0]; s.mu = v[1]; s.mv = v[2]; s.E = v[3];
     return s;
 }
@@ -345,47 +374,49 @@ void apply_bc(Oracle& o) {
 
 // Cell::comImmersedGhostPhase -> ImagePoint::InterpolatePhase
 // (Cell.cpp:172-181, ImagePoint.cpp:20-82)
-void ib_fill(Oracle& o) {
+void ib_fill_one(Oracle& o, const ImmersedGhost& gcell) {
     const double dx = o.dx, dy = o.dy;
-    for (const ImmersedGhost& gcell : o.ig) {
-        if (o.ib_interp == CFS_IB_BILINEAR_POSITIVE) {
-            double val[4];
-            for (int q = 0; q < 4; ++q) {
-                double f[4];
-                for (int d = 0; d < 4; ++d) f[d] = o.U[(size_t)q * o.nx * o.ny + gcell.donor[d]];
-                val[q] = gcell.w[0] * f[0] + gcell.w[1] * f[1] + gcell.w[2] * f[2] + gcell.w[3] * f[3];
-            }
-            const int i = gcell.cell / o.ny, j = gcell.cell % o.ny;
-            o.U[o.at(0, i, j)] = val[0];
-            o.U[o.at(1, i, j)] = -val[1];
-            o.U[o.at(2, i, j)] = -val[2];
-            o.U[o.at(3, i, j)] = val[3];
-            continue;
-        }
-        double A[4][4];
-        double f[4][4];   // [var][donor]
-        for (int d = 0; d < 4; ++d) {
-            const int c = gcell.donor[d];
-            const int i = c / o.ny, j = c % o.ny;
-            const double px = (i + 0.5) * dx + o.cfg.x0;
-            const double py = (j + 0.5) * dy + o.cfg.y0;
-            A[d][0] = px * py; A[d][1] = px; A[d][2] = py; A[d][3] = 1.0;
-            for (int q = 0; q < 4; ++q) f[q][d] = o.U[o.at(q, i, j)];
-        }
-        int tr[4];
-        ib_ldlt_factor(A, tr);
+    if (o.ib_interp == CFS_IB_BILINEAR_POSITIVE) {
         double val[4];
         for (int q = 0; q < 4; ++q) {
-            double w[4] = {f[q][0], f[q][1], f[q][2], f[q][3]};
-            ib_ldlt_solve(A, tr, w);
-            val[q] = w[0] * gcell.ip.x * gcell.ip.y + w[1] * gcell.ip.x + w[2] * gcell.ip.y + w[3];
+            double f[4];
+            for (int d = 0; d < 4; ++d) f[d] = o.U[(size_t)q * o.nx * o.ny + gcell.donor[d]];
+            val[q] = gcell.w[0] * f[0] + gcell.w[1] * f[1] + gcell.w[2] * f[2] + gcell.w[3] * f[3];
         }
         const int i = gcell.cell / o.ny, j = gcell.cell % o.ny;
         o.U[o.at(0, i, j)] = val[0];
         o.U[o.at(1, i, j)] = -val[1];
         o.U[o.at(2, i, j)] = -val[2];
         o.U[o.at(3, i, j)] = val[3];
+        return;
     }
+    double A[4][4];
+    double f[4][4];   // [var][donor]
+    for (int d = 0; d < 4; ++d) {
+        const int c = gcell.donor[d];
+        const int i = c / o.ny, j = c % o.ny;
+        const double px = (i + 0.5) * dx + o.cfg.x0;
+        const double py = (j + 0.5) * dy + o.cfg.y0;
+        A[d][0] = px * py; A[d][1] = px; A[d][2] = py; A[d][3] = 1.0;
+        for (int q = 0; q < 4; ++q) f[q][d] = o.U[o.at(q, i, j)];
+    }
+    int tr[4];
+    ib_ldlt_factor(A, tr);
+    double val[4];
+    for (int q = 0; q < 4; ++q) {
+        double w[4] = {f[q][0], f[q][1], f[q][2], f[q][3]};
+        ib_ldlt_solve(A, tr, w);
+        val[q] = w[0] * gcell.ip.x * gcell.ip.y + w[1] * gcell.ip.x + w[2] * gcell.ip.y + w[3];
+    }
+    const int i = gcell.cell / o.ny, j = gcell.cell % o.ny;
+    o.U[o.at(0, i, j)] = val[0];
+    o.U[o.at(1, i, j)] = -val[1];
+    o.U[o.at(2, i, j)] = -val[2];
+    o.U[o.at(3, i, j)] = val[3];
+}
+
+void ib_fill(Oracle& o) {
+    for (const ImmersedGhost& gcell : o.ig) ib_fill_one(o, gcell);
 }
 
 // TVD2rdRK::advance (TVD2rdRK.cpp:3-31), TVD3rdRK::advance (TVD3rdRK.cpp:3-48)
@@ -435,19 +466,22 @@ const double kZetaR[3][4] = {
 // Any cell index, including corners outside both axes: periodic axes wrap;
 // otherwise the cell is a ghost of the first side found (x before y):
 // Dirichlet = the fixed state, Neumann = the interior cell at the clamped index.
+// (In the multi-rank mode a periodic index is not wrapped: the rank's frame
+// holds the wrapped copy at the unwrapped index.)
 void cell_any(const Oracle& o, int i, int j, double v[4]) {
     int side = -1;
-    if (i < 0) { if (o.periodic_x()) i += o.nx; else side = 0; }
-    else if (i >= o.nx) { if (o.periodic_x()) i -= o.nx; else side = 1; }
-    if (j < 0) { if (o.periodic_y()) j += o.ny; else if (side < 0) side = 2; }
-    else if (j >= o.ny) { if (o.periodic_y()) j -= o.ny; else if (side < 0) side = 3; }
+    const bool wrap = tl_view == nullptr;
+    if (i < 0) { if (o.periodic_x()) { if (wrap) i += o.nx; } else side = 0; }
+    else if (i >= o.nx) { if (o.periodic_x()) { if (wrap) i -= o.nx; } else side = 1; }
+    if (j < 0) { if (o.periodic_y()) { if (wrap) j += o.ny; } else if (side < 0) side = 2; }
+    else if (j >= o.ny) { if (o.periodic_y()) { if (wrap) j -= o.ny; } else if (side < 0) side = 3; }
     if (side >= 0 && o.cfg.bc[side] == CFS_BC_DIRICHLET) {
         for (int q = 0; q < 4; ++q) v[q] = o.cfg.bc_state[side][q];
         return;
     }
-    i = i < 0 ? 0 : (i >= o.nx ? o.nx - 1 : i);
-    j = j < 0 ? 0 : (j >= o.ny ? o.ny - 1 : j);
-    for (int q = 0; q < 4; ++q) v[q] = o.U[o.at(q, i, j)];
+    if (!o.periodic_x()) i = i < 0 ? 0 : (i >= o.nx ? o.nx - 1 : i);
+    if (!o.periodic_y()) j = j < 0 ? 0 : (j >= o.ny ? o.ny - 1 : j);
+    for (int q = 0; q < 4; ++q) v[q] = ucell(o, q, i, j);
 }
 
 struct Prim { double u, v, T; };
@@ -695,6 +729,149 @@ void stage(Oracle& o, int counter, double dt) {
     apply_bc(o);   // Run.cpp:203
 }
 
+// ---------------------------------------------------------------------------
+// Multi-rank restatement of the reference's MPI CPU path
+// (source/parallel/CartesianParallel.cpp): the grid is cut into dims[0] x
+// dims[1] blocks (ZoneDcomposition, :3-47, with MPI_Dims_create's balanced
+// factorisation; a remainder of cells is spread over the first blocks, where
+// the reference's N / dims drops it), one worker thread per block (a rank).
+// Each stage every rank
+//   1. fills the immersed ghosts of its block (Run.cpp:156-162),
+//   2. copies its block and the neighbours' edge strips into a frame of
+//      3-cell-deep ghost cells (InsertCpuBoundaryGhostCells, :60-170, keeps
+//      3 per process boundary; exchangedata, :172-427, refreshes them every
+//      stage) -- all ranks wait for every copy before any cell changes,
+//   3. computes the faces of its block from the frame, the process-boundary
+//      faces on both sides (each rank walks its own ghost chain, as the
+//      reference does), and updates its cells.
+// The physical-boundary fill follows.  Every face sees the same stencil
+// values as in stage(), so the result is bit-identical to it for every
+// decomposition.
+
+class Barrier {
+public:
+    explicit Barrier(int n) : n_(n) {}
+    void wait() {
+        std::unique_lock<std::mutex> lk(m_);
+        const int gen = gen_;
+        if (++count_ == n_) {
+            count_ = 0;
+            ++gen_;
+            cv_.notify_all();
+        } else {
+            cv_.wait(lk, [&] { return gen != gen_; });
+        }
+    }
+
+private:
+    std::mutex m_;
+    std::condition_variable cv_;
+    int n_, count_ = 0, gen_ = 0;
+};
+
+// MPI_Dims_create(n, 2): the two factors of n closest to each other, larger first
+void dims_create(int n, int dims[2]) {
+    int a = 1;
+    for (int f = 1; f * f <= n; ++f)
+        if (n % f == 0) a = f;
+    dims[0] = n / a;
+    dims[1] = a;
+}
+
+void stage_ranks(Oracle& o, int counter, double dt) {
+    const int nx = o.nx, ny = o.ny;
+    const int d0 = o.dims[0], d1 = o.dims[1];
+    const int sub = rk_substage(o.cfg.rk, counter);
+    Barrier bar(d0 * d1);
+    auto rank_main = [&](int cx, int cy) {
+        const int bx0 = (int)((long long)cx * nx / d0), bx1 = (int)((long long)(cx + 1) * nx / d0);
+        const int by0 = (int)((long long)cy * ny / d1), by1 = (int)((long long)(cy + 1) * ny / d1);
+        const int bnx = bx1 - bx0, bny = by1 - by0;
+        // 1. immersed ghosts of the block
+        for (const ImmersedGhost& gc : o.ig) {
+            const int i = gc.cell / ny, j = gc.cell % ny;
+            if (i >= bx0 && i < bx1 && j >= by0 && j < by1) ib_fill_one(o, gc);
+        }
+        bar.wait();
+        // 2. exchange: block + 3-deep frame (periodic axes wrap; past a
+        // physical edge the frame is unused: those reads go to the boundary ghosts)
+        const int fx = bnx + 6, fy = bny + 6;
+        std::vector<double> frame(4 * (size_t)fx * fy, 0.0);
+        for (int a = 0; a < fx; ++a)
+            for (int b = 0; b < fy; ++b) {
+                int i = bx0 - 3 + a, j = by0 - 3 + b;
+                if (o.periodic_x()) i = (i % nx + nx) % nx;
+                if (o.periodic_y()) j = (j % ny + ny) % ny;
+                if (i < 0 || i >= nx || j < 0 || j >= ny) continue;
+                for (int q = 0; q < 4; ++q) frame[(size_t)q * fx * fy + (size_t)a * fy + b] = o.U[o.at(q, i, j)];
+            }
+        bar.wait();   // every frame is filled before any cell of U changes
+        const View v{frame.data(), (size_t)fx * fy, fy, bx0 - 3, by0 - 3};
+        // the frame holds wrapped copies: read them at unwrapped indices
+        tl_view = &v;
+        // 3. faces of the block (x: f = bx0..bx1, y: g = by0..by1), then the update
+        std::vector<double> FX((size_t)(bnx + 1) * bny * 4), FY((size_t)bnx * (bny + 1) * 4);
+        int idx[6];
+        Phase st[6];
+        for (int j = by0; j < by1; ++j)
+            for (int f = bx0; f <= bx1; ++f) {
+                collect_stencil_rank(f, nx, o.periodic_x(), idx);
+                for (int k = 0; k < 6; ++k) st[k] = cell_phase_x(o, idx[k], j);
+                double* F = &FX[((size_t)(f - bx0) * bny + (j - by0)) * 4];
+                face_flux(o, st, true, F);
+                if (o.cfg.positivity == 2) limit_face(o, st[2], st[3], idx[2], j, idx[3], j, true, dt, F);
+                if (o.cfg.viscous) {
+                    double Fv[4];
+                    viscous_flux(o, idx, j, true, Fv);
+                    for (int q = 0; q < 4; ++q) F[q] = F[q] - Fv[q];
+                }
+            }
+        for (int i = bx0; i < bx1; ++i)
+            for (int gf = by0; gf <= by1; ++gf) {
+                collect_stencil_rank(gf, ny, o.periodic_y(), idx);
+                for (int k = 0; k < 6; ++k) st[k] = cell_phase_y(o, i, idx[k]);
+                double* F = &FY[((size_t)(i - bx0) * (bny + 1) + (gf - by0)) * 4];
+                face_flux(o, st, false, F);
+                if (o.cfg.positivity == 2) limit_face(o, st[2], st[3], i, idx[2], i, idx[3], false, dt, F);
+                if (o.cfg.viscous) {
+                    double Fv[4];
+                    viscous_flux(o, idx, i, false, Fv);
+                    for (int q = 0; q < 4; ++q) F[q] = F[q] - Fv[q];
+                }
+            }
+        tl_view = nullptr;
+        for (int i = bx0; i < bx1; ++i)
+            for (int j = by0; j < by1; ++j) {
+                // global face arrays: each face written by one rank (the one
+                // left of / below it, the last rank for the far edge)
+                for (int q = 0; q < 4; ++q) {
+                    o.FX[((size_t)i * ny + j) * 4 + q] = FX[((size_t)(i - bx0) * bny + (j - by0)) * 4 + q];
+                    o.FY[((size_t)i * (ny + 1) + j) * 4 + q] = FY[((size_t)(i - bx0) * (bny + 1) + (j - by0)) * 4 + q];
+                    if (i == nx - 1)
+                        o.FX[((size_t)nx * ny + j) * 4 + q] = FX[((size_t)bnx * bny + (j - by0)) * 4 + q];
+                    if (j == ny - 1)
+                        o.FY[((size_t)i * (ny + 1) + ny) * 4 + q] = FY[((size_t)(i - bx0) * (bny + 1) + bny) * 4 + q];
+                }
+                if (o.type[(size_t)i * ny + j] != CFS_CELL_FLUID) continue;
+                const double* fl = &FX[((size_t)(i - bx0) * bny + (j - by0)) * 4];
+                const double* fr = &FX[((size_t)(i + 1 - bx0) * bny + (j - by0)) * 4];
+                const double* gb = &FY[((size_t)(i - bx0) * (bny + 1) + (j - by0)) * 4];
+                const double* gt = &FY[((size_t)(i - bx0) * (bny + 1) + (j + 1 - by0)) * 4];
+                for (int q = 0; q < 4; ++q) {
+                    const double rhs = -(fr[q] - fl[q]) / o.dx - (gt[q] - gb[q]) / o.dy;
+                    const size_t a = o.at(q, i, j);
+                    o.RHS[a] = rhs;
+                    rk_update(o.cfg.rk, sub, o.U[a], o.snap0[a], o.snap1[a], rhs, dt);
+                }
+            }
+    };
+    std::vector<std::thread> th;
+    for (int cx = 0; cx < d0; ++cx)
+        for (int cy = 0; cy < d1; ++cy) th.emplace_back(rank_main, cx, cy);
+    for (auto& t : th) t.join();
+    apply_bc(o);   // Run.cpp:203
+}
+
 // CFL wave-speed bound (no reference counterpart): max over fluid cells of
 // (max(|u|,|rho u|)+c)/dx + (max(|v|,|rho v|)+c)/dy with c = sqrt(gamma p / rho)
 // (the momentum terms bound the reference HLL's outer wave speeds).
@@ -865,6 +1042,20 @@ oracle_solver* oracle_create(const cfs_config* cfg, int nthreads) {
 
 void oracle_destroy(oracle_solver* s) { delete s; }
 
+// Multi-rank mode with n ranks (1: off); dims_out receives the decomposition.
+int oracle_set_ranks(oracle_solver* s, int n, int* dims_out) {
+    if (!s || n < 1) return CFS_E_ARG;
+    Oracle& o = s->o;
+    int d[2];
+    dims_create(n, d);
+    if (d[0] > o.nx || d[1] > o.ny) return CFS_E_ARG;
+    o.ranks = n;
+    o.dims[0] = d[0];
+    o.dims[1] = d[1];
+    if (dims_out) { dims_out[0] = d[0]; dims_out[1] = d[1]; }
+    return 0;
+}
+
 // Initial state: fills all cells, then (like Run::solver's prologue,
 // Run.cpp:106-132) the immersed-ghost fill and the boundary fill.
 int oracle_set_state(oracle_solver* s, const double* U) {
@@ -940,7 +1131,8 @@ int oracle_get_ib(oracle_solver* s, uint8_t* cell_type, int32_t* ig_cells, int32
 
 int oracle_stage(oracle_solver* s, int counter, double dt) {
     if (counter != s->o.counter) return CFS_E_STATE;
-    stage(s->o, counter, dt);
+    if (s->o.ranks > 1) stage_ranks(s->o, counter, dt);
+    else stage(s->o, counter, dt);
     s->o.counter++;
     return 0;
 }

@@ -33,6 +33,8 @@ def lib():
         getattr(L, n).argtypes = [vp, vp]
         getattr(L, n).restype = ctypes.c_int
     L.oracle_get_face_flux.argtypes = [vp, vp, vp]
+    L.oracle_set_ranks.argtypes = [vp, ctypes.c_int, vp]
+    L.oracle_set_ranks.restype = ctypes.c_int
     L.oracle_viscous_tables.argtypes = [vp, vp, vp]
     L.oracle_set_circle.argtypes = [vp, dbl, dbl, dbl, ctypes.c_int, ctypes.c_int]
     L.oracle_set_ib_table.argtypes = [vp, ctypes.c_int, vp, ctypes.c_int, vp, vp, vp, vp]
@@ -93,6 +95,16 @@ class Oracle:
     @property
     def shape(self):
         return (4, self.cfg.nx, self.cfg.ny)
+
+    def set_ranks(self, n):
+        """Multi-rank mode: n ranks in a 2D block decomposition with 3-cell
+        ghost frames exchanged every stage (the reference's MPI path);
+        returns the decomposition (dims[0], dims[1])."""
+        d = (ctypes.c_int * 2)()
+        rc = self.L.oracle_set_ranks(self.h, n, d)
+        if rc != 0:
+            raise RuntimeError("oracle_set_ranks: %d" % rc)
+        return d[0], d[1]
 
     def set_circle(self, cx, cy, r, npts=400, layers=3):
         rc = self.L.oracle_set_circle(self.h, cx, cy, r, npts, layers)

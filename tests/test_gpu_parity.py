@@ -271,6 +271,24 @@ def test_fp32_bulk_l1_density():
     assert l1 <= 1e-4
 
 
+@pytest.mark.parametrize("rk", [3, 2])
+def test_riemann_400_full_size_100_steps(rk):
+    # the Lax-Liu configuration at its full 400^2 size: 100 steps at the CFL
+    # time step of the initial state, every field bit-identical (the contract
+    # is L-inf <= 1e-11 relative after 100 steps)
+    cfg = cfs.config_riemann(400, rk=rk)
+    U = cfs.riemann_quadrants(cfg)
+    s = cfs.Solver(cfg)
+    s.set_state(U)
+    o = O.Oracle(cfg, nthreads=8)
+    o.set_state(U)
+    dt = 0.5 / o.cfl_smax()
+    s.advance(100, dt=dt)
+    for n in range(100):
+        o.step(dt)
+    _assert_parity(s.get_state(), o.get_state(), "riemann 400^2 rk%d 100 steps" % rk)
+
+
 def test_full_size_riemann_symmetry():
     # 400^2, the Lax-Liu configuration: symmetric about x = y, bit for bit
     cfg = cfs.config_riemann(400)
