@@ -138,6 +138,17 @@ int launch_step(cfs_solver* s, bool timed, bool use_cfl, double cfl, double t_en
     return CFS_OK;
 }
 
+// steps between two reads of the device halt flag in a CFL advance
+constexpr int HALT_POLL = 64;
+
+int read_halt(cfs_solver* s, bool* halted) {
+    int h = 0;
+    HIP_OK(hipMemcpyAsync(&h, scal(s).halt(), sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    *halted = h != 0;
+    return CFS_OK;
+}
+
 int set_dt_device(cfs_solver* s, double dt) {
     hipLaunchKernelGGL(set_scalar_kernel, dim3(1), dim3(1), 0, s->stream, scal(s).dt(), dt);
     HIP_OK(hipGetLastError());
@@ -427,14 +438,27 @@ int32_t cfs_advance(cfs_solver* s, int32_t nsteps, double dt, double cfl, double
                 s->counter = counter0;   // capture launched nothing
             }
             // the captured launches left step/stage parities and planes where two
-            // real steps leave them
+            // real steps leave them.  Every HALT_POLL steps the halt flag is read
+            // back: once the end time is reached (or the bound is not finite)
+            // the remaining steps would do nothing, so they are not launched.
             for (; n + 2 <= nsteps; n += 2) {
+                if (n > 0 && n % HALT_POLL == 0) {
+                    bool halted = false;
+                    if ((rc = read_halt(s, &halted))) return rc;
+                    if (halted) break;
+                }
                 HIP_OK(hipGraphLaunch(s->graph_exec, s->stream));
                 s->counter += 2 * s->cfg.rk;
             }
             s->cur = 0;
+            if (n + 2 <= nsteps) n = nsteps;   // stopped at a halt
         }
         for (; n < nsteps; ++n) {
+            if (n > 0 && n % HALT_POLL == 0) {
+                bool halted = false;
+                if ((rc = read_halt(s, &halted))) return rc;
+                if (halted) break;
+            }
             rc = launch_step(s, timed, true, cfl, t_end);
             if (rc) return rc;
         }

@@ -235,19 +235,27 @@ __device__ __forceinline__ R weno5(R v0, R v1, R v2, R v3, R v4) {
     return (w0 * q0 + w1 * q1) + w2 * q2;
 }
 
-// Face flux from the 6-cell stencil in the face-normal frame.  S holds the
-// staged state as four planes of ns cells (rho, rho u, rho v, E); cell[k] is
-// the LDS cell of stencil offset k - 3, read as (rho, normal momentum,
-// tangential momentum, E).  Left state: WENO over cells 0..4; right state:
-// over cells 5..1 (Weno5th.cpp:57-81).  Pressure (EulerPhase.cpp:130-133),
-// Euler flux (EulerMod.cpp:33-54) and HLL (RiemmanSolverHLL.cpp:5-75) follow.
-// The x and y forms of the reference map onto this one form because IEEE
-// addition and multiplication are commutative (mx*my == my*mx bit for bit).
-// out = (mass, normal momentum, tangential momentum, energy) flux.
+// The divisions of a face state (r, n, t, E) by its density, as the
+// reference computes them: pressure p = gm1 (E - (n^2 + t^2)/2 / r)
+// (EulerPhase.cpp:132), u_n = n / r, n n / r, n t / r (EulerMod.cpp:35-50) and
+// g p / r (RiemmanSolverHLL.cpp:18-23).
+template <typename R>
+__device__ __forceinline__ void state_terms(R r, R n, R t, R E, R g, R gm1, R& p, R& un, R& nn, R& nt, R& gp) {
+    using C = K<R>;
+    p = gm1 * (E - C::div(C::half * (n * n + t * t), r));
+    un = C::div(n, r);
+    nn = C::div(n * n, r);
+    nt = C::div(n * t, r);
+    gp = C::div(g * p, r);
+}
+
+template <typename R>
+__device__ __forceinline__ void riemann_flux(const R L[4], const R Rr[4], const R cl[4], const R cr[4], R g,
+                                             int positivity, int hll_bounds, R out[4]);
+
 template <typename R, bool FAST>
 __device__ __forceinline__ void face_flux(const R* __restrict__ S, int ns, const int cell[6], int nrm, int tng,
                                           R g, int positivity, int hll_bounds, R out[4]) {
-    using C = K<R>;
     R L[4], Rr[4], cl[4], cr[4];
     const int comp[4] = {0, nrm, tng, 3};
 #pragma unroll
@@ -260,29 +268,40 @@ __device__ __forceinline__ void face_flux(const R* __restrict__ S, int ns, const
         cl[q] = v[2];
         cr[q] = v[3];
     }
+    riemann_flux<R>(L, Rr, cl, cr, g, positivity, hll_bounds, out);
+}
+
+// Pressures, the positivity fallback, Euler fluxes and the HLL flux of a
+// face's two reconstructed states (cl, cr: the adjacent cells' states).
+template <typename R>
+__device__ __forceinline__ void riemann_flux(const R L[4], const R Rr[4], const R cl[4], const R cr[4], R g,
+                                             int positivity, int hll_bounds, R out[4]) {
+    using C = K<R>;
     const R gm1 = g - R(1);
     R rL = L[0], nL = L[1], tL = L[2], EL = L[3];
     R rR = Rr[0], nR = Rr[1], tR = Rr[2], ER = Rr[3];
-    R pL = gm1 * (EL - C::div(C::half * (nL * nL + tL * tL), rL));
-    R pR = gm1 * (ER - C::div(C::half * (nR * nR + tR * tR), rR));
+    // pressure and the state's divisions by its density (EulerPhase.cpp:132,
+    // EulerMod.cpp:35-50, RiemmanSolverHLL.cpp:18-23)
+    R pL, unL, nnL, ntL, gpL, pR, unR, nnR, ntR, gpR;
+    state_terms<R>(rL, nL, tL, EL, g, gm1, pL, unL, nnL, ntL, gpL);
+    state_terms<R>(rR, nR, tR, ER, g, gm1, pR, unR, nnR, ntR, gpR);
     if (positivity) {
         // an inadmissible reconstruction takes the state of the cell on its side
         if (!(rL > R(0) && pL > R(0))) {
             rL = cl[0]; nL = cl[1]; tL = cl[2]; EL = cl[3];
-            pL = gm1 * (EL - C::div(C::half * (nL * nL + tL * tL), rL));
+            state_terms<R>(rL, nL, tL, EL, g, gm1, pL, unL, nnL, ntL, gpL);
         }
         if (!(rR > R(0) && pR > R(0))) {
             rR = cr[0]; nR = cr[1]; tR = cr[2]; ER = cr[3];
-            pR = gm1 * (ER - C::div(C::half * (nR * nR + tR * tR), rR));
+            state_terms<R>(rR, nR, tR, ER, g, gm1, pR, unR, nnR, ntR, gpR);
         }
     }
     // Euler fluxes of both states
-    const R unL = C::div(nL, rL), unR = C::div(nR, rR);
-    const R FL0 = nL, FL1 = C::div(nL * nL, rL) + pL, FL2 = C::div(nL * tL, rL), FL3 = (EL + pL) * unL;
-    const R FR0 = nR, FR1 = C::div(nR * nR, rR) + pR, FR2 = C::div(nR * tR, rR), FR3 = (ER + pR) * unR;
+    const R FL0 = nL, FL1 = nnL + pL, FL2 = ntL, FL3 = (EL + pL) * unL;
+    const R FR0 = nR, FR1 = nnR + pR, FR2 = ntR, FR3 = (ER + pR) * unR;
     // HLL with PVRS star-state speed estimates
-    const R aL = C::sqrt_(C::div(g * pL, rL));
-    const R aR = C::sqrt_(C::div(g * pR, rR));
+    const R aL = C::sqrt_(gpL);
+    const R aR = C::sqrt_(gpR);
     const R a_ave = C::half * (aL + aR);
     const R rho_ave = C::half * (rL + rR);
     const R p_star = C::half * (pL + pR) - (C::half * (unR - unL)) * rho_ave * a_ave;
@@ -480,31 +499,41 @@ template <typename R, typename Sh> struct LimitLDS {
 
 template <typename R> __device__ __forceinline__ R maxr(R x, R y) { return (x < y) ? y : x; }
 
+// density times pressure without a division: rho p = gm1 (E rho - (n^2 + t^2)/2)
 template <typename R>
-__device__ __forceinline__ R pressure_n(R r, R n, R t, R E, R gm1) {
-    return gm1 * (E - K<R>::div(K<R>::half * (n * n + t * t), r));
+__device__ __forceinline__ R rho_p(R r, R n, R t, R E, R gm1) {
+    return gm1 * (E * r - K<R>::half * (n * n + t * t));
 }
 
 // largest theta in [0, 1] keeping B0 + theta*Dl admissible: density first,
-// then pressure, which is concave along the density-limited segment
+// then pressure, which is concave along the density-limited segment.
+// Floors: min(1e-13, Lax-Friedrichs value) in FP64; in FP32 a thousandth of
+// the Lax-Friedrichs value, which stays clear of the rounding of FP32 fluxes.
+// The pressure test p1 < floor is made on rho p, multiplied out (positive
+// densities), so the common case -- nothing to limit -- has no division.
 template <typename R>
 __device__ __forceinline__ R share_theta(const R B0[4], const R Dl[4], R gm1) {
     using C = K<R>;
     const R r0 = B0[0];
-    const R p0 = pressure_n<R>(B0[0], B0[1], B0[2], B0[3], gm1);
-    // floors: min(1e-13, Lax-Friedrichs value) in FP64; in FP32 a thousandth of
-    // the Lax-Friedrichs value, which stays clear of the rounding of FP32 fluxes
+    const R P0 = rho_p<R>(B0[0], B0[1], B0[2], B0[3], gm1);
     const R eps = R(1e-13);
     const R er = (sizeof(R) == 8) ? ((r0 < eps) ? r0 : eps) : r0 * R(1e-3);
-    const R ep = (sizeof(R) == 8) ? ((p0 < eps) ? p0 : eps) : p0 * R(1e-3);
     R th = R(1);
     const R r1 = B0[0] + Dl[0];
     if (r1 < er) th = C::div(r0 - er, r0 - r1);
     R B1[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) B1[q] = B0[q] + th * Dl[q];
-    const R p1 = pressure_n<R>(B1[0], B1[1], B1[2], B1[3], gm1);
-    if (p1 < ep) th = th * C::div(p0 - ep, p0 - p1);
+    const R P1 = rho_p<R>(B1[0], B1[1], B1[2], B1[3], gm1);
+    // FP64: p1 < min(p0, eps), i.e. p0 < eps ? p1 < p0 : p1 < eps; FP32: p1 < p0/1000
+    const bool low0 = P0 < eps * r0;
+    const bool below = (sizeof(R) == 8) ? (low0 ? (P1 * r0 < P0 * B1[0]) : (P1 < eps * B1[0]))
+                                        : (P1 * r0 < (R(1e-3) * P0) * B1[0]);
+    if (below) {
+        const R p0 = C::div(P0, r0), p1 = C::div(P1, B1[0]);
+        const R ep = (sizeof(R) == 8) ? ((p0 < eps) ? p0 : eps) : p0 * R(1e-3);
+        th = th * C::div(p0 - ep, p0 - p1);
+    }
     return th;
 }
 

@@ -303,40 +303,55 @@ def main_riemann(args):
     print(json.dumps(out))
 
 
-def run_kh(precision, n, t_end, chunk=1000):
+def run_kh(precision, n, t_end, chunk=1000, checkpoints=(), perturb=0.0):
     """The viscous shear layer from its seeded initial state to t_end with CFL
-    steps; progress every chunk steps.  Returns timings and the density field."""
+    steps (progress every chunk steps), stopping exactly at each checkpoint
+    time to record the density.  perturb > 0 multiplies the initial density
+    by (1 + perturb) (a control for the flow's own sensitivity)."""
     cfg = cfs.config_kelvin_helmholtz(n, precision=precision)
     s = cfs.Solver(cfg)
-    s.set_state(cfs.kelvin_helmholtz(cfg))
+    U = cfs.kelvin_helmholtz(cfg)
+    if perturb:
+        U = U.copy()
+        U[0] *= 1.0 + perturb
+    s.set_state(U)
     t, steps, status = 0.0, 0, "ok"
+    snaps = {}
     t0 = time.perf_counter()
-    while t < t_end:
-        try:
-            t, done = s.advance(chunk, dt=0.0, cfl=CFL, t_end=t_end, t=t)
-        except cfs.CfsError as e:
-            status = str(e)
+    for stop in sorted(set(list(checkpoints) + [t_end])):
+        while t < stop:
+            try:
+                t, done = s.advance(chunk, dt=0.0, cfl=CFL, t_end=stop, t=t)
+            except cfs.CfsError as e:
+                status = str(e)
+                break
+            steps += done
+            print("kh %s%s: %d steps, t = %.4f, %.1f s" % (
+                "fp64" if precision == cfs.PREC_FP64 else "fp32-bulk", " (perturbed)" if perturb else "", steps,
+                t, time.perf_counter() - t0), file=sys.stderr, flush=True)
+            if done == 0:
+                break
+        if status != "ok":
             break
-        steps += done
-        print("kh %s: %d steps, t = %.4f, %.1f s" % ("fp64" if precision == cfs.PREC_FP64 else "fp32-bulk", steps,
-                                                    t, time.perf_counter() - t0), file=sys.stderr, flush=True)
-        if done == 0:
-            break
+        snaps[stop] = s.get_state()[0]
     wall = time.perf_counter() - t0
-    rho = s.get_state()[0]
+    rho = snaps.get(t_end, s.get_state()[0])
     s.set_kernel_timing(True)
     s.advance(5, dt=0.0, cfl=CFL, t_end=0.0, t=t)
     kms = s.last_kernel_ms()
     s.close()
     return dict(steps=steps, t=t, wall=wall, mcups=n * n * 3 * steps / wall / 1e6, kernel_ms=kms, rho=rho,
-                status=status, finite=bool(np.all(np.isfinite(rho))))
+                snaps=snaps, status=status, finite=bool(np.all(np.isfinite(rho))))
 
 
 def main_kh(args):
     n, t_end = args.kh_n, args.kh_t_end
-    r64 = run_kh(cfs.PREC_FP64, n, t_end)
-    r32 = run_kh(cfs.PREC_FP32_BULK, n, t_end)
+    cps = (0.1, 0.25, 0.5) if args.kh_control else ()
+    r64 = run_kh(cfs.PREC_FP64, n, t_end, checkpoints=cps)
+    r32 = run_kh(cfs.PREC_FP32_BULK, n, t_end, checkpoints=cps)
     l1 = float(np.abs(r32["rho"] - r64["rho"]).mean())
+    strip = lambda r: {k: (round(v, 5) if isinstance(v, float) else v) for k, v in r.items()
+                       if k not in ("rho", "snaps")}
     out = {
         "metric": "Mcell-updates/s per RK stage (viscous shear layer)",
         "value": round(r64["mcups"], 2),
@@ -347,12 +362,20 @@ def main_kh(args):
         "data": "synthetic (seeded shear layer, noise seed %d)" % cfs.KH_SEED,
         "config": {"workload": "kelvin_helmholtz_%dx%d_viscous_re1e5_pr0.72_rk3_cfl0.5_t%g" % (n, n, t_end),
                    "periodic": True},
-        "fp64": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in r64.items() if k != "rho"},
-        "fp32_bulk": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in r32.items() if k != "rho"},
+        "fp64": strip(r64),
+        "fp32_bulk": strip(r32),
         "fp32_bulk_speedup": round(r32["mcups"] / r64["mcups"], 3),
         "l1_density_fp32_bulk_vs_fp64": l1,
         "linf_density_fp32_bulk_vs_fp64": float(np.abs(r32["rho"] - r64["rho"]).max()),
     }
+    if args.kh_control:
+        # the flow's own sensitivity: FP64 from a density perturbed by 1e-14
+        # (relative), compared with the FP64 run at the same times
+        rp = run_kh(cfs.PREC_FP64, n, t_end, checkpoints=cps, perturb=1e-14)
+        out["l1_density_vs_time"] = [
+            {"t": tc, "fp32_bulk_vs_fp64": float(np.abs(r32["snaps"][tc] - r64["snaps"][tc]).mean()),
+             "fp64_perturbed_1e-14_vs_fp64": float(np.abs(rp["snaps"][tc] - r64["snaps"][tc]).mean())}
+            for tc in sorted(r64["snaps"]) if tc in r32["snaps"] and tc in rp["snaps"]]
     print(json.dumps(out))
 
 
@@ -442,6 +465,8 @@ def main():
                     help="cylinder: the headline line; batch / serve: the batched-solves and serving measurements")
     ap.add_argument("--kh-n", type=int, default=4096, help="shear-layer grid side (--workload kh)")
     ap.add_argument("--kh-t-end", type=float, default=1.0, help="shear-layer end time (--workload kh)")
+    ap.add_argument("--kh-control", action="store_true",
+                    help="also record L1(rho) at t = 0.1, 0.25, 0.5 and run FP64 from a 1e-14-perturbed density")
     ap.add_argument("--serve-count", type=int, default=48)
     ap.add_argument("--serve-rates", default="1,2,4,0", help="offered request rates per second (0: burst)")
     ap.add_argument("--serve-arena-gb", type=int, default=4)

@@ -574,8 +574,9 @@ void viscous_flux(const Oracle& o, const int idx[6], int line, bool xdir, double
 
 double maxr(double x, double y) { return (x < y) ? y : x; }
 
-double pressure_n(double g, double r, double n, double t, double E) {
-    return (g - 1) * (E - 0.5 * (n * n + t * t) / r);
+// density times pressure without a division: rho p = (g-1) (E rho - (n^2 + t^2)/2)
+double rho_p(double g, double r, double n, double t, double E) {
+    return (g - 1) * (E * r - 0.5 * (n * n + t * t));
 }
 
 // one division: ir = 1/r, un = n*ir, ut = t*ir, p = (g-1)*(E - 0.5*(n*un + t*ut))
@@ -602,19 +603,26 @@ double wave_speed(const Oracle& o, int i, int j, bool xdir) {
     return (u < 0.0 ? -u : u) + c;
 }
 
+// the pressure test p1 < min(p0, 1e-13) is made on rho p, multiplied out
+// (positive densities); the pressures themselves only when limiting
 double share_theta(double g, const double B0[4], const double Dl[4]) {
     const double eps = 1e-13;
     const double r0 = B0[0];
-    const double p0 = pressure_n(g, B0[0], B0[1], B0[2], B0[3]);
+    const double P0 = rho_p(g, B0[0], B0[1], B0[2], B0[3]);
     const double er = (r0 < eps) ? r0 : eps;
-    const double ep = (p0 < eps) ? p0 : eps;
     double th = 1.0;
     const double r1 = B0[0] + Dl[0];
     if (r1 < er) th = (r0 - er) / (r0 - r1);
     double B1[4];
     for (int q = 0; q < 4; ++q) B1[q] = B0[q] + th * Dl[q];
-    const double p1 = pressure_n(g, B1[0], B1[1], B1[2], B1[3]);
-    if (p1 < ep) th = th * ((p0 - ep) / (p0 - p1));
+    const double P1 = rho_p(g, B1[0], B1[1], B1[2], B1[3]);
+    const bool low0 = P0 < eps * r0;
+    const bool below = low0 ? (P1 * r0 < P0 * B1[0]) : (P1 < eps * B1[0]);
+    if (below) {
+        const double p0 = P0 / r0, p1 = P1 / B1[0];
+        const double ep = (p0 < eps) ? p0 : eps;
+        th = th * ((p0 - ep) / (p0 - p1));
+    }
     return th;
 }
 
