@@ -499,42 +499,58 @@ template <typename R, typename Sh> struct LimitLDS {
 
 template <typename R> __device__ __forceinline__ R maxr(R x, R y) { return (x < y) ? y : x; }
 
-// density times pressure without a division: rho p = gm1 (E rho - (n^2 + t^2)/2)
 template <typename R>
-__device__ __forceinline__ R rho_p(R r, R n, R t, R E, R gm1) {
-    return gm1 * (E * r - K<R>::half * (n * n + t * t));
+__device__ __forceinline__ R pressure_n(R r, R n, R t, R E, R gm1) {
+    return gm1 * (E - K<R>::div(K<R>::half * (n * n + t * t), r));
+}
+
+// density times pressure without a division: rho p = gm1 (E rho - (n^2 + t^2)/2)
+__device__ __forceinline__ double rho_p(double r, double n, double t, double E, double gm1) {
+    return gm1 * (E * r - 0.5 * (n * n + t * t));
 }
 
 // largest theta in [0, 1] keeping B0 + theta*Dl admissible: density first,
 // then pressure, which is concave along the density-limited segment.
 // Floors: min(1e-13, Lax-Friedrichs value) in FP64; in FP32 a thousandth of
 // the Lax-Friedrichs value, which stays clear of the rounding of FP32 fluxes.
-// The pressure test p1 < floor is made on rho p, multiplied out (positive
+// FP64 makes the pressure test p1 < floor on rho p, multiplied out (positive
 // densities), so the common case -- nothing to limit -- has no division.
 template <typename R>
 __device__ __forceinline__ R share_theta(const R B0[4], const R Dl[4], R gm1) {
     using C = K<R>;
     const R r0 = B0[0];
-    const R P0 = rho_p<R>(B0[0], B0[1], B0[2], B0[3], gm1);
     const R eps = R(1e-13);
-    const R er = (sizeof(R) == 8) ? ((r0 < eps) ? r0 : eps) : r0 * R(1e-3);
-    R th = R(1);
-    const R r1 = B0[0] + Dl[0];
-    if (r1 < er) th = C::div(r0 - er, r0 - r1);
-    R B1[4];
+    if constexpr (sizeof(R) == 8) {
+        const double P0 = rho_p(B0[0], B0[1], B0[2], B0[3], gm1);
+        const double er = (r0 < eps) ? r0 : eps;
+        double th = 1.0;
+        const double r1 = B0[0] + Dl[0];
+        if (r1 < er) th = C::div(r0 - er, r0 - r1);
+        double B1[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) B1[q] = B0[q] + th * Dl[q];
-    const R P1 = rho_p<R>(B1[0], B1[1], B1[2], B1[3], gm1);
-    // FP64: p1 < min(p0, eps), i.e. p0 < eps ? p1 < p0 : p1 < eps; FP32: p1 < p0/1000
-    const bool low0 = P0 < eps * r0;
-    const bool below = (sizeof(R) == 8) ? (low0 ? (P1 * r0 < P0 * B1[0]) : (P1 < eps * B1[0]))
-                                        : (P1 * r0 < (R(1e-3) * P0) * B1[0]);
-    if (below) {
-        const R p0 = C::div(P0, r0), p1 = C::div(P1, B1[0]);
-        const R ep = (sizeof(R) == 8) ? ((p0 < eps) ? p0 : eps) : p0 * R(1e-3);
-        th = th * C::div(p0 - ep, p0 - p1);
+        for (int q = 0; q < 4; ++q) B1[q] = B0[q] + th * Dl[q];
+        const double P1 = rho_p(B1[0], B1[1], B1[2], B1[3], gm1);
+        // p1 < min(p0, eps): p0 < eps ? p1 < p0 : p1 < eps
+        const bool below = (P0 < eps * r0) ? (P1 * r0 < P0 * B1[0]) : (P1 < eps * B1[0]);
+        if (below) {
+            const double p0 = C::div(P0, r0), p1 = C::div(P1, B1[0]);
+            const double ep = (p0 < eps) ? p0 : eps;
+            th = th * C::div(p0 - ep, p0 - p1);
+        }
+        return th;
+    } else {
+        const R p0 = pressure_n<R>(B0[0], B0[1], B0[2], B0[3], gm1);
+        const R er = r0 * R(1e-3), ep = p0 * R(1e-3);
+        R th = R(1);
+        const R r1 = B0[0] + Dl[0];
+        if (r1 < er) th = C::div(r0 - er, r0 - r1);
+        R B1[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) B1[q] = B0[q] + th * Dl[q];
+        const R p1 = pressure_n<R>(B1[0], B1[1], B1[2], B1[3], gm1);
+        if (p1 < ep) th = th * C::div(p0 - ep, p0 - p1);
+        return th;
     }
-    return th;
 }
 
 // Euler flux of a cell state in the face-normal frame, one division: with
