@@ -207,8 +207,36 @@ __device__ __forceinline__ bool fast_div_safe(double v) {
 // WENO5 value at the face from five cell values, v2 the upwind cell
 // (source/interpolation/Weno5th.cpp:4-28: Jiang-Shu smoothness indicators,
 // eps = 1e-2, ideal weights 0.1/0.6/0.3).  Operation order as the reference.
+//
+// FP32-bulk evaluates the same formula with fused multiply-adds and one
+// reciprocal of the weight sum (the value is a convex combination either way;
+// FP32 results are compared with FP64 within a tolerance, not bit for bit):
+// (a0 q0 + a1 q1 + a2 q2) / (a0 + a1 + a2), a_k = d_k / (eps + beta_k)^2.
+__device__ __forceinline__ float weno5_f32(float v0, float v1, float v2, float v3, float v4) {
+    using C = K<float>;
+    const float s0 = __builtin_fmaf(-2.0f, v1, v0) + v2;
+    const float t0 = __builtin_fmaf(3.0f, v2, __builtin_fmaf(-4.0f, v1, v0));
+    const float b0 = __builtin_fmaf(C::c1312 * s0, s0, (0.25f * t0) * t0);
+    const float s1 = __builtin_fmaf(-2.0f, v2, v1) + v3;
+    const float t1 = v1 - v3;
+    const float b1 = __builtin_fmaf(C::c1312 * s1, s1, (0.25f * t1) * t1);
+    const float s2 = __builtin_fmaf(-2.0f, v3, v2) + v4;
+    const float t2 = __builtin_fmaf(-4.0f, v3, __builtin_fmaf(3.0f, v2, v4));
+    const float b2 = __builtin_fmaf(C::c1312 * s2, s2, (0.25f * t2) * t2);
+    const float e0 = C::eps + b0, e1 = C::eps + b1, e2 = C::eps + b2;
+    const float a0 = C::d0 * __builtin_amdgcn_rcpf(e0 * e0);
+    const float a1 = C::d1 * __builtin_amdgcn_rcpf(e1 * e1);
+    const float a2 = C::d2 * __builtin_amdgcn_rcpf(e2 * e2);
+    const float q0 = __builtin_fmaf(11.0f / 6.0f, v2, __builtin_fmaf(-7.0f / 6.0f, v1, v0 * (1.0f / 3.0f)));
+    const float q1 = __builtin_fmaf(1.0f / 3.0f, v3, __builtin_fmaf(5.0f / 6.0f, v2, v1 * (-1.0f / 6.0f)));
+    const float q2 = __builtin_fmaf(-1.0f / 6.0f, v4, __builtin_fmaf(5.0f / 6.0f, v3, v2 * (1.0f / 3.0f)));
+    const float num = __builtin_fmaf(a2, q2, __builtin_fmaf(a1, q1, a0 * q0));
+    return num * __builtin_amdgcn_rcpf((a0 + a1) + a2);
+}
+
 template <typename R, bool FAST>
 __device__ __forceinline__ R weno5(R v0, R v1, R v2, R v3, R v4) {
+    if constexpr (sizeof(R) == 4) return weno5_f32(v0, v1, v2, v3, v4);
     using C = K<R>;
     using D = WOps<R, FAST>;
     const R s0 = D::sub_pow2(v0, R(2), v1) + v2;
