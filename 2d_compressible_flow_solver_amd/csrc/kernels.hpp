@@ -118,6 +118,8 @@ template <> struct K<double> {
     static constexpr double d0 = 0.1, d1 = 0.6, d2 = 0.3, half = 0.5;
     __device__ static double div(double a, double b) { return a / b; }
     __device__ static double sqrt_(double a) { return __builtin_sqrt(a); }
+    // a*b + c rounded twice, as written in the reference (no contraction)
+    __device__ static double madd(double a, double b, double c) { return a * b + c; }
 };
 template <> struct K<float> {
     static constexpr float eps = 1e-2f, c1312 = 13.0f / 12.0f, q = 0.25f;
@@ -125,6 +127,8 @@ template <> struct K<float> {
     // FP32-bulk: hardware reciprocal (1 ulp) instead of the IEEE division
     __device__ static float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
     __device__ static float sqrt_(float a) { return __builtin_amdgcn_sqrtf(a); }
+    // FP32-bulk fuses a*b + c
+    __device__ static float madd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 };
 
 // Divisions of the WENO5 reconstruction.  WOps<double, true> is the fast form
@@ -270,7 +274,7 @@ __device__ __forceinline__ R weno5(R v0, R v1, R v2, R v3, R v4) {
 template <typename R>
 __device__ __forceinline__ void state_terms(R r, R n, R t, R E, R g, R gm1, R& p, R& un, R& nn, R& nt, R& gp) {
     using C = K<R>;
-    p = gm1 * (E - C::div(C::half * (n * n + t * t), r));
+    p = gm1 * (E - C::div(C::half * C::madd(n, n, t * t), r));
     un = C::div(n, r);
     nn = C::div(n * n, r);
     nt = C::div(n * t, r);
@@ -332,7 +336,7 @@ __device__ __forceinline__ void riemann_flux(const R L[4], const R Rr[4], const 
     const R aR = C::sqrt_(gpR);
     const R a_ave = C::half * (aL + aR);
     const R rho_ave = C::half * (rL + rR);
-    const R p_star = C::half * (pL + pR) - (C::half * (unR - unL)) * rho_ave * a_ave;
+    const R p_star = C::madd(-((C::half * (unR - unL)) * rho_ave), a_ave, C::half * (pL + pR));
     const R u_star = C::half * (unR + unL) - C::div(C::half * (pR - pL), rho_ave * a_ave);
     const R rsL = rL + C::div((unL - u_star) * rho_ave, a_ave);
     const R rsR = rR - C::div((unR - u_star) * rho_ave, a_ave);
@@ -345,10 +349,10 @@ __device__ __forceinline__ void riemann_flux(const R L[4], const R Rr[4], const 
     const R SR = (y1 < y2) ? y2 : y1;           // std::max
     const R inv = C::div(R(1), SR - SL);
     const R ss = SL * SR;
-    const R H0 = ((SR * FL0 - SL * FR0) + ss * (rR - rL)) * inv;
-    const R H1 = ((SR * FL1 - SL * FR1) + ss * (nR - nL)) * inv;
-    const R H2 = ((SR * FL2 - SL * FR2) + ss * (tR - tL)) * inv;
-    const R H3 = ((SR * FL3 - SL * FR3) + ss * (ER - EL)) * inv;
+    const R H0 = C::madd(ss, rR - rL, C::madd(SR, FL0, -(SL * FR0))) * inv;
+    const R H1 = C::madd(ss, nR - nL, C::madd(SR, FL1, -(SL * FR1))) * inv;
+    const R H2 = C::madd(ss, tR - tL, C::madd(SR, FL2, -(SL * FR2))) * inv;
+    const R H3 = C::madd(ss, ER - EL, C::madd(SR, FL3, -(SL * FR3))) * inv;
     const bool left = SL >= R(0), right = SR <= R(0);
     out[0] = left ? FL0 : right ? FR0 : H0;
     out[1] = left ? FL1 : right ? FR1 : H1;
@@ -529,7 +533,7 @@ template <typename R> __device__ __forceinline__ R maxr(R x, R y) { return (x < 
 
 template <typename R>
 __device__ __forceinline__ R pressure_n(R r, R n, R t, R E, R gm1) {
-    return gm1 * (E - K<R>::div(K<R>::half * (n * n + t * t), r));
+    return gm1 * (E - K<R>::div(K<R>::half * K<R>::madd(n, n, t * t), r));
 }
 
 // density times pressure without a division: rho p = gm1 (E rho - (n^2 + t^2)/2)
@@ -556,7 +560,7 @@ __device__ __forceinline__ R share_theta(const R B0[4], const R Dl[4], R gm1) {
         if (r1 < er) th = C::div(r0 - er, r0 - r1);
         double B1[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) B1[q] = B0[q] + th * Dl[q];
+        for (int q = 0; q < 4; ++q) B1[q] = C::madd(th, Dl[q], B0[q]);
         const double P1 = rho_p(B1[0], B1[1], B1[2], B1[3], gm1);
         // p1 < min(p0, eps): p0 < eps ? p1 < p0 : p1 < eps
         const bool below = (P0 < eps * r0) ? (P1 * r0 < P0 * B1[0]) : (P1 < eps * B1[0]);
@@ -574,7 +578,7 @@ __device__ __forceinline__ R share_theta(const R B0[4], const R Dl[4], R gm1) {
         if (r1 < er) th = C::div(r0 - er, r0 - r1);
         R B1[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) B1[q] = B0[q] + th * Dl[q];
+        for (int q = 0; q < 4; ++q) B1[q] = C::madd(th, Dl[q], B0[q]);
         const R p1 = pressure_n<R>(B1[0], B1[1], B1[2], B1[3], gm1);
         if (p1 < ep) th = th * C::div(p0 - ep, p0 - p1);
         return th;
@@ -588,9 +592,9 @@ __device__ __forceinline__ void euler_flux_n(R r, R n, R t, R E, R gm1, R F[4]) 
     using C = K<R>;
     const R ir = C::div(R(1), r);
     const R un = n * ir, ut = t * ir;
-    const R p = gm1 * (E - C::half * (n * un + t * ut));
+    const R p = gm1 * (E - C::half * C::madd(n, un, t * ut));
     F[0] = n;
-    F[1] = n * un + p;
+    F[1] = C::madd(n, un, p);
     F[2] = n * ut;
     F[3] = (E + p) * un;
 }
@@ -613,25 +617,25 @@ __device__ __forceinline__ void limit_face(const R* __restrict__ S, int ns, cons
     R Flf[4], D[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        Flf[q] = C::half * (FL[q] + FR[q]) - (C::half * a) * (UR[q] - UL[q]);
+        Flf[q] = C::madd(-(C::half * a), UR[q] - UL[q], C::half * (FL[q] + FR[q]));
         D[q] = fl[q] - Flf[q];
     }
     const R anL = maxr<R>(maxr<R>(sn[cL - dn], sn[cL]), sn[cL + dn]);
     const R atL = maxr<R>(maxr<R>(st[cL - dtg], st[cL]), st[cL + dtg]);
     const R anR = maxr<R>(maxr<R>(sn[cR - dn], sn[cR]), sn[cR + dn]);
     const R atR = maxr<R>(maxr<R>(st[cR - dtg], st[cR]), st[cR + dtg]);
-    const R kL = C::div(R(2) * (ln * anL + lt * atL), anL);
-    const R kR = C::div(R(2) * (ln * anR + lt * atR), anR);
+    const R kL = C::div(R(2) * C::madd(ln, anL, lt * atL), anL);
+    const R kR = C::div(R(2) * C::madd(ln, anR, lt * atR), anR);
     R B0[4], Dl[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        B0[q] = UL[q] - kL * (Flf[q] - FL[q]);
+        B0[q] = C::madd(-kL, Flf[q] - FL[q], UL[q]);
         Dl[q] = -(kL * D[q]);
     }
     const R thL = share_theta<R>(B0, Dl, gm1);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        B0[q] = UR[q] + kR * (Flf[q] - FR[q]);
+        B0[q] = C::madd(kR, Flf[q] - FR[q], UR[q]);
         Dl[q] = kR * D[q];
     }
     const R thR = share_theta<R>(B0, Dl, gm1);
@@ -639,7 +643,7 @@ __device__ __forceinline__ void limit_face(const R* __restrict__ S, int ns, cons
     th = (th < R(0)) ? R(0) : th;
     if (th < R(1)) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) fl[q] = Flf[q] + th * D[q];
+        for (int q = 0; q < 4; ++q) fl[q] = C::madd(th, D[q], Flf[q]);
     }
 }
 
