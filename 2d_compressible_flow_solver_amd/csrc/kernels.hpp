@@ -7,12 +7,12 @@
 // LDS holds the staged state and the face fluxes as planes per variable
 // (structure of arrays), so the lanes of a wave read consecutive doubles.
 //
-// Stage kernel (one launch per RK stage): a 256-thread workgroup owns a
-// TI x TJ cell tile (15 x 16 with 256 threads by default, larger shapes
-// below).  It stages the tile plus a 3-cell halo
-// (the WENO5 stencil reach, minus the unused corners) in LDS, applying the
-// physical boundary on the fly (one ghost cell per boundary face, as
-// source/boundary/*Boundary.cpp; periodic wrap), then
+// Stage kernel (one launch per RK stage): a workgroup owns a TI x TJ cell
+// tile (15 x 16 with 256 threads by default; the other tile shapes are
+// below).  It stages the tile plus a 3-cell halo (the WENO5 stencil reach,
+// minus the unused corners) in LDS, applying the physical boundary on the fly
+// (one ghost cell per boundary face, as source/boundary/*Boundary.cpp;
+// periodic wrap), then
 //   phase 1: every thread computes one or two faces -- the 16x16 = 256
 //            x-faces and the 15x17 = 255 y-faces of the tile, 511 in all,
 //            two rounds of 256 with one idle slot -- WENO5 left/right
@@ -26,8 +26,11 @@
 //
 // Arithmetic: FP64 kernels reproduce the reference's operations one for one
 // in its source order (compiled with -ffp-contract=off; IEEE division and
-// square root), so their results equal a strict-IEEE CPU evaluation of the
-// reference source bit for bit.
+// square root, or forms proven to give the same bits -- WOps), so their
+// results equal a strict-IEEE CPU evaluation of the reference source bit for
+// bit.  FP32-bulk (R = float) evaluates the reconstruction, fluxes and RHS in
+// FP32 with fused multiply-adds and hardware reciprocals; the state, the RK
+// combination, the immersed-boundary fill and the CFL bound stay FP64.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
