@@ -194,6 +194,27 @@ def run_batch(precision, count, poll_every=20, seed=BATCH_SEED, max_steps=20000)
                       file=sys.stderr, flush=True)
         wall = time.perf_counter() - t0
     failed = sum(1 for st, _, _ in res.values() if st == cfs.SLOT_FAILED)
+    # accuracy of the batched FP32-bulk results: the first 16 problems again in
+    # FP64 through the single-solve path, L1(rho) at the same end time
+    l1 = []
+    if precision == cfs.PREC_FP32_BULK:
+        with cfs.Batch(rk=2, precision=precision, max_problems=16, limiter=True, tile_shape=shape,
+                       arena_bytes=int(sum(cfs.Batch.footprint(c) for c, _ in probs[:16]) * 1.05) + (64 << 20)) as b2:
+            slots = {b2.admit(c, u, cfl=CFL, t_end=BATCH_T_END): k for k, (c, u) in enumerate(probs[:16])}
+            got = {}
+            while len(got) < len(slots):
+                b2.step(poll_every)
+                for slot, (state, steps, t) in b2.poll().items():
+                    if state != cfs.SLOT_RUNNING and slots[slot] not in got:
+                        got[slots[slot]] = b2.retire(slot)
+        for k, (c, u) in enumerate(probs[:16]):
+            c64 = cfs.Config.from_buffer_copy(c)
+            c64.precision = cfs.PREC_FP64
+            s = cfs.Solver(c64)
+            s.set_state(u)
+            s.advance(max_steps, dt=0.0, cfl=CFL, t_end=BATCH_T_END)
+            l1.append(float(np.abs(got[k][0] - s.get_state()[0]).mean()))
+            s.close()
     stalled = sum(1 for st, _, _ in res.values() if st == cfs.SLOT_RUNNING)
     cell_stages = sum(3 * res[k][1] * probs[k][0].nx * probs[k][0].ny for k in range(count))
     # the same problems one after another, each alone
@@ -216,7 +237,7 @@ def run_batch(precision, count, poll_every=20, seed=BATCH_SEED, max_steps=20000)
                 mcups=cell_stages / wall / 1e6, solves_per_s=count / wall, failed=failed, stalled=stalled,
                 seq_wall=seq_wall, seq_mcups=cell_stages / seq_wall / 1e6, seq_solves_per_s=count / seq_wall,
                 steps_min=min(r[1] for r in res.values()), steps_max=max(r[1] for r in res.values()),
-                shape=shape, shape_ms=shape_ms)
+                shape=shape, shape_ms=shape_ms, l1=l1)
 
 
 def main_batch(args):
@@ -243,6 +264,9 @@ def main_batch(args):
                               "solves_per_s": round(r["seq_solves_per_s"], 2)},
         "speedup_vs_one_after_another": round(r["seq_wall"] / r["wall"], 3),
     }
+    if r["l1"]:
+        out["l1_density_vs_fp64"] = {"problems": len(r["l1"]), "median": float(np.median(r["l1"])),
+                                     "max": float(np.max(r["l1"]))}
     print(json.dumps(out))
 
 
@@ -266,6 +290,43 @@ def main_riemann(args):
     gpu_wall = time.perf_counter() - t0
     G = s.get_state()
     s.close()
+    # FP32-bulk on the same problem to the same end time.  The reference's
+    # TVD3rdRK as written is not stable at CFL 0.5 here: rounding noise grows
+    # until the state is non-finite (an FP64 run from a density perturbed by
+    # 1e-12 shows it), so FP32-bulk is compared with FP64 on the reference
+    # driver's TVD2rdRK (Run.cpp:47), stable at CFL 0.5.
+    def solve(precision, rk, perturb=0.0):
+        c = cfs.config_riemann(400, rk=rk, precision=precision)
+        s = cfs.Solver(c)
+        V0 = U
+        if perturb:
+            V0 = U.copy()
+            V0[0] = V0[0] * (1.0 + perturb * np.sin(np.arange(V0[0].size, dtype=np.float64)).reshape(V0[0].shape))
+        s.set_state(V0)
+        t0 = time.perf_counter()
+        t, n, status = 0.0, 0, "ok"
+        while t < 0.3:
+            try:
+                t, done = s.advance(10, dt=0.0, cfl=CFL, t_end=0.3, t=t)
+            except cfs.CfsError:
+                status = "non-finite within steps %d-%d (t = %.4f at step %d)" % (n + 1, n + 10, t, n)
+                break
+            n += done
+            if done == 0:
+                break
+        wall = time.perf_counter() - t0
+        V = s.get_state()
+        s.close()
+        return dict(steps=n, t=round(t, 6), wall_s=round(wall, 4), status=status), V
+    rk3_32, _ = solve(cfs.PREC_FP32_BULK, 3)
+    rk3_pert, _ = solve(cfs.PREC_FP64, 3, perturb=1e-12)
+    r2_64, H64 = solve(cfs.PREC_FP64, 2)
+    r2_32, H32 = solve(cfs.PREC_FP32_BULK, 2)
+    rk2 = {"fp64": r2_64, "fp32_bulk": r2_32}
+    if r2_64["status"] == "ok" and r2_32["status"] == "ok":
+        rk2["l1_density_fp32_bulk_vs_fp64"] = float(np.abs(H32[0] - H64[0]).mean())
+        rk2["linf_density_fp32_bulk_vs_fp64"] = float(np.abs(H32[0] - H64[0]).max())
+
     nthreads = max(1, min(16, os.cpu_count() or 1))
     o = O.Oracle(cfg, nthreads=nthreads)
     o.set_ranks(nthreads)
@@ -294,6 +355,9 @@ def main_riemann(args):
         "data": "Lax-Liu configuration 3 quadrant states (no randomness)",
         "config": {"workload": "riemann_laxliu3_400x400_rk3_cfl0.5_t0.3_fp64"},
         "gpu": {"steps": steps, "t": t, "wall_s": round(gpu_wall, 4)},
+        "rk3_fp32_bulk": rk3_32,
+        "rk3_fp64_density_perturbed_1e-12": rk3_pert,
+        "rk2_fp32_bulk_vs_fp64": rk2,
         "cpu_baseline": {"value": round(cells * 3 * n / cpu_wall / 1e6, 3), "unit": "Mcell-updates/s",
                          "cores": nthreads, "kind": "port", "steps": n, "wall_s": round(cpu_wall, 3),
                          "sample": "the whole solve to t = 0.3"},
