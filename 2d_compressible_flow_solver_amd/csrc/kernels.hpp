@@ -105,11 +105,18 @@ struct StageArgs {
 };
 
 // LDS of the viscous variant: primitive u, v, T of every staged cell and the
-// cell gradients d/dy (x-face cross terms) and d/dx (y-face cross terms) of u, v
+// cell gradients d/dy (x-face cross terms) and d/dx (y-face cross terms) of u,
+// v where the faces read them.  An x-face at tile row fr reads d/dy at staged
+// rows fr + 1 .. fr + 4 of its own column, so d/dy is kept for staged rows
+// 1 .. TI + 4 of the tile's TJ columns (index (row - 1) * TJ + col - HALO);
+// likewise d/dx for the tile's TI rows and staged columns 1 .. TJ + 4 (index
+// (row - HALO) * (TJ + 4) + col - 1).
 template <typename R, typename Sh> struct ViscLDS {
     static constexpr int N = Sh::SR * Sh::SC;
+    static constexpr int GYW = Sh::TJ, NGY = (Sh::TI + 4) * Sh::TJ;        // d/dy: rows x columns
+    static constexpr int GXW = Sh::TJ + 4, NGX = Sh::TI * (Sh::TJ + 4);    // d/dx
     R pu[N], pv[N], pT[N];
-    R gyu[N], gyv[N], gxu[N], gxv[N];
+    R gyu[NGY], gyv[NGY], gxu[NGX], gxv[NGX];
 };
 
 // ---------------------------------------------------------------------------
@@ -683,10 +690,11 @@ template <> struct VT<float> {
 };
 
 // net viscous flux (0, normal-momentum, tangential-momentum, energy) of a face
-// whose 6 stencil cells have LDS cell indices cs[k]
+// whose 6 stencil cells have LDS cell indices cs[k]; gs[m] indexes the
+// gradient arrays at cell cs[1 + m]
 template <typename R, typename Sh>
-__device__ __forceinline__ void viscous_flux(const ViscLDS<R, Sh>& V, const int cs[6], bool xface, R h, R mu, R kappa,
-                                             R out[3]) {
+__device__ __forceinline__ void viscous_flux(const ViscLDS<R, Sh>& V, const int cs[6], const int gs[4], bool xface,
+                                             R h, R mu, R kappa, R out[3]) {
     const R* un = xface ? V.pu : V.pv;
     const R* ut = xface ? V.pv : V.pu;
     const R* gn = xface ? V.gyu : V.gxv;   // tangential derivative of the normal velocity
@@ -707,8 +715,8 @@ __device__ __forceinline__ void viscous_flux(const ViscLDS<R, Sh>& V, const int 
         R dt_un = R(0), dt_ut = R(0), unf = R(0), utf = R(0);
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
-            dt_un += VT<R>::zr[r][m] * gn[cs[1 + m]];
-            dt_ut += VT<R>::zr[r][m] * gt[cs[1 + m]];
+            dt_un += VT<R>::zr[r][m] * gn[gs[m]];
+            dt_ut += VT<R>::zr[r][m] * gt[gs[m]];
             unf += VT<R>::zr[r][m] * un[cs[1 + m]];
             utf += VT<R>::zr[r][m] * ut[cs[1 + m]];
         }
@@ -736,7 +744,7 @@ __device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, d
     // Lax-Friedrichs shares of the limiter: dt/dx and dt/dy
     const R lam_x = (R)(dt / a.dx), lam_y = (R)(dt / a.dy);
     for (int s = tid; s < NXF + NYF; s += BLOCK) {
-        int cell[6], nrm, tng;
+        int cell[6], gcell[4], nrm, tng;
         if (s < NXF) {
             const int fr = s / TJ, jl = s - fr * TJ;
             const int f = i0 + fr;
@@ -748,6 +756,7 @@ __device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, d
                 if (!px && f == 0 && o > 0) o = 0;
                 if (!px && f == a.nx && o < -1) o = -1;
                 cell[k] = (fr + o + HALO) * SC + jl + HALO;
+                if (VISC && k >= 1 && k <= 4) gcell[k - 1] = (fr + o + HALO - 1) * ViscLDS<R, Sh>::GYW + jl;
             }
             nrm = 1; tng = 2;
         } else {
@@ -760,6 +769,7 @@ __device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, d
                 if (!py && gf == 0 && o > 0) o = 0;
                 if (!py && gf == a.ny && o < -1) o = -1;
                 cell[k] = (il + HALO) * SC + gl + o + HALO;
+                if (VISC && k >= 1 && k <= 4) gcell[k - 1] = il * ViscLDS<R, Sh>::GXW + gl + o + HALO - 1;
             }
             nrm = 2; tng = 1;
         }
@@ -778,7 +788,7 @@ __device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, d
             for (int k = 0; k < 6; ++k) cs[k] = cell[k];
             const bool xface = s < NXF;
             R fv[3];
-            viscous_flux<R, Sh>(*V, cs, xface, xface ? (R)a.dx : (R)a.dy, (R)a.mu, (R)a.kappa, fv);
+            viscous_flux<R, Sh>(*V, cs, gcell, xface, xface ? (R)a.dx : (R)a.dy, (R)a.mu, (R)a.kappa, fv);
             fl[1] = fl[1] - fv[0];
             fl[2] = fl[2] - fv[1];
             fl[3] = fl[3] - fv[2];
@@ -901,29 +911,34 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
             V->pT[s] = K<R>::div(gma2 * p, rho);
         }
         __syncthreads();
-        // 4th-order central cell gradients wherever the 5-cell stencil is staged
+        // 4th-order central cell gradients where the faces read them (the
+        // 5-cell stencils stay inside the staged region: d/dy columns
+        // HALO +- 2 .. HALO + TJ - 1 + 2, d/dx rows likewise)
+        using VL = ViscLDS<R, Sh>;
         const R rdx = (R)a.dx, rdy = (R)a.dy;
-        for (int s = tid; s < SR * SC; s += BLOCK) {
-            const int r = s / SC, c = s - r * SC;
-            if (c >= 2 && c < SC - 2) {
-                R du = R(0), dv = R(0);
+        for (int q = tid; q < VL::NGY + VL::NGX; q += BLOCK) {
+            R du = R(0), dv = R(0);
+            if (q < VL::NGY) {
+                const int gr = q / VL::GYW, gc = q - gr * VL::GYW;
+                const int s = (gr + 1) * SC + gc + HALO;
 #pragma unroll
                 for (int k = 0; k < 5; ++k) {
                     du += VT<R>::thc[k] * V->pu[s - 2 + k];
                     dv += VT<R>::thc[k] * V->pv[s - 2 + k];
                 }
-                V->gyu[s] = K<R>::div(du, rdy);
-                V->gyv[s] = K<R>::div(dv, rdy);
-            }
-            if (r >= 2 && r < SR - 2) {
-                R du = R(0), dv = R(0);
+                V->gyu[q] = K<R>::div(du, rdy);
+                V->gyv[q] = K<R>::div(dv, rdy);
+            } else {
+                const int t = q - VL::NGY;
+                const int gr = t / VL::GXW, gc = t - gr * VL::GXW;
+                const int s = (gr + HALO) * SC + gc + 1;
 #pragma unroll
                 for (int k = 0; k < 5; ++k) {
                     du += VT<R>::thc[k] * V->pu[s + (k - 2) * SC];
                     dv += VT<R>::thc[k] * V->pv[s + (k - 2) * SC];
                 }
-                V->gxu[s] = K<R>::div(du, rdx);
-                V->gxv[s] = K<R>::div(dv, rdx);
+                V->gxu[t] = K<R>::div(du, rdx);
+                V->gxv[t] = K<R>::div(dv, rdx);
             }
         }
         __syncthreads();
@@ -986,16 +1001,16 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
 }
 
 // Workgroups per CU the kernels are compiled for (VGPR budget): waves per CU
-// of 20 (FP64), 24 (FP32), 16 with the limiter (its LDS), 8 viscous (57 KB of
-// LDS per 256 threads in FP64), divided by the shape's waves per workgroup.
+// of 20 (FP64), 24 (FP32), 16 with the limiter (its LDS), 12 viscous (52 KB
+// of LDS per 256 threads in FP64), divided by the shape's waves per workgroup.
 template <typename R, bool VISC, bool LIMIT, typename Sh> struct Occupancy {
-    static constexpr int waves = VISC ? 8 : (LIMIT ? 16 : (sizeof(R) == 8 ? 20 : 24));
+    static constexpr int waves = VISC ? (sizeof(R) == 8 ? 12 : 20) : (LIMIT ? 16 : (sizeof(R) == 8 ? 20 : 24));
     static constexpr int blocks = waves / (Sh::NT / 64) > 0 ? waves / (Sh::NT / 64) : 1;
 };
 
 // The stage kernel: one workgroup per tile.  With 15 x 16 tiles: FP64 at
 // most 96 VGPRs so that five workgroups (20 waves) share a CU; FP32 six; the
-// viscous variant's LDS (57 KB in FP64) allows two, the limiter's (38 KB) four.
+// viscous variant's LDS (52 KB in FP64) allows three, the limiter's (38 KB) four.
 //
 // Tile order: workgroups are dispatched round-robin over the 8 XCDs
 // (workgroup b to XCD b % 8), each with its own L2.  The grid is 8 x ceil(n/8)
