@@ -56,6 +56,53 @@ def test_weno_non_oscillatory_at_a_step():
     assert 1.0 - 1e-3 < r <= 1.0
 
 
+
+def _weno5_exact(v):
+    """Weno5th.cpp:4-28 evaluated in exact rational arithmetic on the FP64
+    constants (eps = 1e-2, d = 0.1, 0.6, 0.3, 13/12 as doubles): the FP64
+    evaluation may differ from it only by rounding."""
+    F = Fraction
+    v0, v1, v2, v3, v4 = (F(x) for x in v)
+    c13, eps = F(13.0 / 12.0), F(1e-2)
+    b0 = c13 * (v0 - 2 * v1 + v2) ** 2 + F(1, 4) * (v0 - 4 * v1 + 3 * v2) ** 2
+    b1 = c13 * (v1 - 2 * v2 + v3) ** 2 + F(1, 4) * (v1 - v3) ** 2
+    b2 = c13 * (v2 - 2 * v3 + v4) ** 2 + F(1, 4) * (3 * v2 - 4 * v3 + v4) ** 2
+    a = [F(0.1) / (eps + b0) ** 2, F(0.6) / (eps + b1) ** 2, F(0.3) / (eps + b2) ** 2]
+    q = [v0 / 3 - 7 * v1 / 6 + 11 * v2 / 6, -v1 / 6 + 5 * v2 / 6 + v3 / 3, v2 / 3 + 5 * v3 / 6 - v4 / 6]
+    return sum(ai * qi for ai, qi in zip(a, q)) / sum(a)
+
+
+def test_weno_within_rounding_of_exact_rational_value():
+    # a dropped term, a wrong sign or a swapped coefficient moves the result by
+    # far more than rounding: the bound is 64 ulp of the largest input
+    rng = np.random.default_rng(11)
+    for k in range(300):
+        scale = 10.0 ** rng.integers(-3, 4)
+        v = (rng.normal(size=5) * scale).tolist()
+        if k % 3 == 0:                    # near-smooth data: small beta
+            v = (v[0] + 1e-3 * scale * np.arange(5) ** 2).tolist()
+        r = O.weno5(v)
+        exact = _weno5_exact(v)
+        tol = 64 * np.spacing(max(abs(x) for x in v))
+        assert abs(Fraction(r) - exact) <= Fraction(tol), (v, r, float(exact))
+
+
+def test_weno_fifth_order_on_smooth_data():
+    # cell averages of sin(x) on cells of width h (2 sin(c) sin(h/2) / h, free
+    # of cancellation); face value at the right edge of the middle cell; with
+    # eps = 1e-2 >> beta ~ h^2 the weights approach the linear ones, so the
+    # error falls as h^5 (ratio 32 per halving).  A scheme of order 3 (a wrong
+    # linear weight) or 4 fails the lower bound.
+    errs = []
+    x0 = 0.3
+    for h in (0.025, 0.0125, 0.00625):
+        centres = x0 + h * (np.arange(5) - 2)
+        avg = [2.0 * math.sin(c) * math.sin(h / 2) / h for c in centres]
+        errs.append(abs(O.weno5(avg) - math.sin(x0 + h / 2)))
+    orders = [math.log2(errs[i] / errs[i + 1]) for i in range(2)]
+    assert all(4.8 < o < 5.5 for o in orders), (errs, orders)
+
+
 # ---- Euler flux and HLL ------------------------------------------------------
 def test_euler_flux_known_answer():
     S = [2.0, 1.0, 0.5, 5.0]           # rho, rho u, rho v, E
@@ -156,6 +203,28 @@ def test_rk3_as_written_ignores_second_stage_rhs():
     b = O.rk_apply(3, 1.0, [0.5, 11.0, 2.0], 0.1)
     assert a == b
     assert a == pytest.approx(1.0 + 2.0 / 3.0 * 0.1 * 0.5 + 2.0 / 3.0 * 0.1 * 2.0, rel=1e-15)
+
+
+
+def _rk_linear(rk, z, u0=1.0):
+    """One step of the oracle's stage formulas on L(U) = lam U with dt = 1,
+    lam = z: the right-hand side of each sub-stage is lam times the state
+    the previous sub-stage produced (rk_apply replays the earlier ones)."""
+    rhs, u = [], u0
+    for k in range(rk):
+        rhs.append(z * u)
+        u = O.rk_apply(rk, u0, rhs, 1.0)
+    return u
+
+
+@pytest.mark.parametrize("z", [-0.9, -0.3, 0.05, 0.4])
+def test_rk_amplification_factors_of_linear_problem(z):
+    # TVD2rdRK: U1 = (1+z)U0, U = U0/2 + (U1 + z U1)/2  =>  R = 1 + z + z^2/2
+    assert _rk_linear(2, z) == pytest.approx(1 + z + z * z / 2, rel=1e-14)
+    # TVD3rdRK as written: U1 = (1+z)U0, U2 = 3/4 U0 + 1/4 (1+z)^2 U0,
+    # U = U0/3 + 2/3 U1 + 2/3 z U2  =>  R = 1 + 4z/3 + z^2/3 + z^3/6
+    # (the SSP scheme's R = 1 + z + z^2/2 + z^3/6)
+    assert _rk_linear(3, z) == pytest.approx(1 + 4 * z / 3 + z * z / 3 + z ** 3 / 6, rel=1e-14)
 
 
 # ---- checksum tree -----------------------------------------------------------

@@ -7,8 +7,12 @@ fill) by invariants the mathematics fixes:
     constant to round-off;
   - a uniform flow with transmissive or matching-Dirichlet boundaries stays
     uniform;
-  - the CFL reduction equals a brute-force evaluation of its definition.
+  - the CFL reduction equals a brute-force evaluation of its definition;
+  - with transmissive boundaries the summed right-hand side equals the net
+    flux through the boundary faces (the interior fluxes telescope).
 """
+import math
+
 import numpy as np
 
 import _oracle as O
@@ -79,3 +83,27 @@ def test_stage_sequence_enforced():
     import pytest
     with pytest.raises(RuntimeError):
         o.stage(2, 1e-3)
+
+
+def test_rhs_sum_equals_boundary_flux():
+    # Cell.cpp:110-125: L = -(F_{i+1/2} - F_{i-1/2})/dx - (G_{j+1/2} - G_{j-1/2})/dy;
+    # summed over the grid only the boundary faces remain.  A wrong sign, a
+    # shifted face index or an unmatched interior flux breaks the identity.
+    cfg = cfs.make_config(37, 29, 1.0, 0.8, rk=3)
+    o = O.Oracle(cfg, nthreads=2)
+    rng = np.random.default_rng(5)
+    U = cfs.riemann_quadrants(cfs.config_riemann(64))[:, :37, :29].copy()
+    U *= 1.0 + 0.05 * rng.random(U.shape)
+    o.set_state(U)
+    dt = 0.5 / o.cfl_smax()
+    for stage in (1, 2, 3):
+        o.stage(stage, dt)
+        L = o.get_rhs()
+        fx, fy = o.face_flux()
+        dx, dy = 1.0 / 37, 0.8 / 29
+        for q in range(4):
+            lhs = math.fsum(L[q].ravel())
+            rhs = -(math.fsum(fx[-1, :, q]) - math.fsum(fx[0, :, q])) / dx \
+                  - (math.fsum(fy[:, -1, q]) - math.fsum(fy[:, 0, q])) / dy
+            scale = np.abs(L[q]).sum()
+            assert abs(lhs - rhs) <= 1e-12 * scale, (stage, q, lhs, rhs)
