@@ -235,7 +235,7 @@ int32_t cfs_batch_create(const cfs_batch_config* bc, void* hip_stream, cfs_batch
         if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) { delete b; return CFS_E_HIP; }
         b->own_stream = true;
     }
-    // persistent grid: every CU filled to the stage kernel's occupancy
+    // persistent grid, sized from the stage kernel's occupancy
     int dev = 0, cus = 0, per_cu = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -246,7 +246,10 @@ int32_t cfs_batch_create(const cfs_batch_config* bc, void* hip_stream, cfs_batch
                  : (lim ? blocks_per_cu_shape<float, true>(k, &per_cu) : blocks_per_cu_shape<float, false>(k, &per_cu));
     }
     if (e != hipSuccess) { cfs_batch_destroy(b); return CFS_E_HIP; }
-    int g = bc->workgroups > 0 ? bc->workgroups : cus * (per_cu > 0 ? per_cu : 1);
+    // four times the workgroups the CUs hold at once: each takes a quarter of
+    // the tiles, and the hardware starts the waiting ones as others finish, so
+    // tiles of uneven cost (body, limiter, halted problems) even out
+    int g = bc->workgroups > 0 ? bc->workgroups : 4 * cus * (per_cu > 0 ? per_cu : 1);
     b->grid = ((g + 7) / 8) * 8;
     // tile list capacity: twice the full tiles an arena of problem planes
     // holds (96 B per cell) plus 16 ragged tiles per slot; admission past it

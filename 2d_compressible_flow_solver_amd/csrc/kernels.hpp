@@ -1000,12 +1000,18 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
     if (a.fuse_cfl) block_max_to(smax, a.acc_out, wmax);
 }
 
-// Workgroups per CU the kernels are compiled for (VGPR budget): waves per CU
-// of 20 (FP64), 24 (FP32), 16 with the limiter (its LDS), 12 viscous (52 KB
-// of LDS per 256 threads in FP64), divided by the shape's waves per workgroup.
+// Waves per SIMD the stage kernels are compiled for: the second argument of
+// __launch_bounds__ is the minimum number of waves per execution unit (SIMD)
+// on this target, which sets the VGPR budget (512 / waves).  The wanted
+// figure -- FP64 5, FP32 6, with the limiter FP64 4 (105 VGPRs), viscous FP64
+// 3 (133 VGPRs) and FP32 5 -- is capped by the waves per SIMD that the
+// workgroup's LDS (160 KB per CU) allows.
 template <typename R, bool VISC, bool LIMIT, typename Sh> struct Occupancy {
-    static constexpr int waves = VISC ? (sizeof(R) == 8 ? 12 : 20) : (LIMIT ? 16 : (sizeof(R) == 8 ? 20 : 24));
-    static constexpr int blocks = waves / (Sh::NT / 64) > 0 ? waves / (Sh::NT / 64) : 1;
+    static constexpr int want = VISC ? (sizeof(R) == 8 ? 3 : 5) : LIMIT ? (sizeof(R) == 8 ? 4 : 6) : (sizeof(R) == 8 ? 5 : 6);
+    static constexpr int extra = VISC ? (int)sizeof(ViscLDS<R, Sh>) : LIMIT ? (int)sizeof(LimitLDS<R, Sh>) : 0;
+    static constexpr int lds = (int)sizeof(R) * (Sh::SR * Sh::SC * 4 + (Sh::NXF + Sh::NYF) * 4) + 8 * (Sh::NT / 64) + extra;
+    static constexpr int by_lds = (160 * 1024 / lds) * (Sh::NT / 64) / 4;
+    static constexpr int waves_per_simd = want < by_lds ? want : (by_lds > 0 ? by_lds : 1);
 };
 
 // The stage kernel: one workgroup per tile.  With 15 x 16 tiles: FP64 at
@@ -1018,7 +1024,7 @@ template <typename R, bool VISC, bool LIMIT, typename Sh> struct Occupancy {
 // one contiguous band of tiles and neighbouring tiles, which share halo
 // cells, are staged through the same L2.
 template <typename R, bool VISC, bool LIMIT, typename Sh>
-__global__ __launch_bounds__(Sh::NT, (Occupancy<R, VISC, LIMIT, Sh>::blocks)) void stage_kernel(StageArgs a) {
+__global__ __launch_bounds__(Sh::NT, (Occupancy<R, VISC, LIMIT, Sh>::waves_per_simd)) void stage_kernel(StageArgs a) {
     __shared__ R S[Sh::SR * Sh::SC * 4];
     __shared__ R F[(Sh::NXF + Sh::NYF) * 4];
     __shared__ double wmax[Sh::NT / 64];
@@ -1042,9 +1048,11 @@ __global__ __launch_bounds__(Sh::NT, (Occupancy<R, VISC, LIMIT, Sh>::blocks)) vo
 // one per XCD (workgroup b is dispatched to XCD b % 8), so tiles that share
 // halo cells are staged through the same L2.  Every problem's arithmetic is
 // the single-solve kernel's, so a problem's result does not depend on what
-// else is in the batch.  The grid size is a multiple of 8.
+// else is in the batch.  The grid size is a multiple of 8.  (Taking tiles
+// from an atomic counter per range instead of the fixed stride was slower:
+// a range's workgroups reach the counter together.)
 template <typename R, bool LIMIT, typename Sh>
-__global__ __launch_bounds__(Sh::NT, (Occupancy<R, false, LIMIT, Sh>::blocks)) void batch_stage_kernel(
+__global__ __launch_bounds__(Sh::NT, (Occupancy<R, false, LIMIT, Sh>::waves_per_simd)) void batch_stage_kernel(
     const StageArgs* __restrict__ descs, const int2* __restrict__ tiles, const int* __restrict__ ntiles, int parity,
     int sub) {
     __shared__ R S[Sh::SR * Sh::SC * 4];

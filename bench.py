@@ -49,6 +49,7 @@ sys.path.insert(0, ROOT)
 cfs = importlib.import_module("2d_compressible_flow_solver_amd")
 
 HBM_PEAK_GBS = 8000.0
+FP64_VALU_PEAK_T = 39.3      # lane-ops/s / 1e12: 78.6 TFLOP/s FP64 vector = 39.3 T FMA lanes/s
 NX, NY, LX, LY = 2048, 1024, 4.0, 2.0
 CX, CY, RADIUS = 1.0, 1.0, 0.125
 CFL = 0.5
@@ -71,15 +72,16 @@ def shape_report(r):
     return out
 
 
-def traffic_from_profiles():
-    """HBM bytes per stage-kernel launch measured with rocprofv3 --pmc
-    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), from the
-    committed summary; None when absent."""
-    p = os.path.join(ROOT, "profiles", "stage_kernel_limiter_fp64_cylinder_mach3_2048x1024_traffic.txt")
+def profile_figure(key):
+    """A figure of the FP64 stage kernel measured with rocprofv3 --pmc, from
+    the committed summary: traffic_bytes_per_launch (FETCH_SIZE + WRITE_SIZE,
+    scaled by the copy-kernel calibration) or valu_lane_ops_per_cell_stage
+    (SQ_INSTS_VALU x 64 / cells); None when absent."""
+    p = os.path.join(ROOT, "profiles", "stage_kernel_limiter_fp64_cylinder_mach3_2048x1024_counters.txt")
     if not os.path.exists(p):
         return None
     for line in open(p):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             return float(line.split(":", 1)[1])
     return None
 
@@ -654,12 +656,19 @@ def main():
                    "ib_interpolation": "bilinear_positive", "finite_at_end": r64["finite"],
                    "tile_shape": shape_report(r64)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profile_figure("traffic_bytes_per_launch"),
                      "kernel": kernel_name(cfs.PREC_FP64, True, r64["shape"]),
                      "kernel_ms": round(r64["kernel_ms"], 5),
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,
     }
+    lane_ops = profile_figure("valu_lane_ops_per_cell_stage")
+    if lane_ops and r64["kernel_ms"] > 0:
+        # the bound that binds: FP64 VALU issue (the kernel's VALU lane-ops per
+        # cell-stage from the committed counters, at the measured launch duration)
+        v = lane_ops * cells / (r64["kernel_ms"] * 1e-3) / 1e12
+        out["roofline"]["valu"] = {"achieved": round(v, 2), "peak": FP64_VALU_PEAK_T, "unit": "T lane-ops/s",
+                                   "frac": round(v / FP64_VALU_PEAK_T, 4), "lane_ops_per_cell_stage": lane_ops}
     if rref is not None:
         out["reference_scheme"] = {"value": round(rref["mcups"], 2), "unit": "Mcell-updates/s",
                                    "kernel": kernel_name(cfs.PREC_FP64, False, rref["shape"]),
