@@ -16,6 +16,7 @@
 
 #include "../../include/cfs.h"
 #include "host.hpp"
+#include "serve_lane.h"
 #include "kernels.hpp"
 
 using namespace cfs;
@@ -133,6 +134,8 @@ struct cfs_batch {
     int2* tiles_host = nullptr;       // pinned staging of the list
     double* scal_host = nullptr;      // pinned copy for polling
     int grid = 0;
+    int resident = 0;                 // workgroups of the stage kernel the CUs hold at once
+    int device = 0;                   // the device current at creation
     int parity = 0;                   // step parity of the next step
     bool dirty = false;               // tile list to rebuild before the next step
     hipGraph_t graph = nullptr;
@@ -249,7 +252,9 @@ int32_t cfs_batch_create(const cfs_batch_config* bc, void* hip_stream, cfs_batch
     // four times the workgroups the CUs hold at once: each takes a quarter of
     // the tiles, and the hardware starts the waiting ones as others finish, so
     // tiles of uneven cost (body, limiter, halted problems) even out
-    int g = bc->workgroups > 0 ? bc->workgroups : 4 * cus * (per_cu > 0 ? per_cu : 1);
+    b->device = dev;
+    b->resident = cus * (per_cu > 0 ? per_cu : 1);
+    int g = bc->workgroups > 0 ? bc->workgroups : 4 * b->resident;
     b->grid = ((g + 7) / 8) * 8;
     // tile list capacity: twice the full tiles an arena of problem planes
     // holds (96 B per cell) plus 16 ragged tiles per slot; admission past it
@@ -298,18 +303,34 @@ void cfs_batch_destroy(cfs_batch* b) {
     delete b;
 }
 
-int32_t cfs_batch_admit(cfs_batch* b, const cfs_config* cfg, const double* U, const cfs_immersed* ib, double cfl,
-                        double t_end, int32_t* slot) {
-    if (!b || !U || !slot) return CFS_E_ARG;
+int32_t batch_check_problem(const cfs_batch* b, const cfs_config* cfg, const cfs_immersed* ib, double cfl,
+                            double t_end) {
+    if (!b) return CFS_E_ARG;
     int rc = check_config(cfg);
     if (rc) return rc;
     if (cfg->rk != b->bc.rk || cfg->precision != b->bc.precision) return CFS_E_ARG;
     if (cfg->viscous) return CFS_E_UNSUPPORTED;
     if (cfg->positivity == 2 && !b->bc.limiter) return CFS_E_ARG;   // the batch runs the kernel without the limiter
     if (!(cfl > 0.0) || !(cfl < 1e300) || t_end != t_end) return CFS_E_ARG;
+    if (ib != nullptr && (ib->n_ig <= 0 || !ib->image_xy || !ib->coef)) return CFS_E_ARG;
+    return CFS_OK;
+}
+
+int64_t batch_problem_tiles(const cfs_batch* b, const cfs_config* cfg) { return tiles_of(b, *cfg); }
+
+int64_t batch_resident_workgroups(const cfs_batch* b) { return b->resident; }
+
+int32_t batch_make_device_current(const cfs_batch* b) {
+    return hipSetDevice(b->device) == hipSuccess ? CFS_OK : CFS_E_HIP;
+}
+
+int32_t cfs_batch_admit(cfs_batch* b, const cfs_config* cfg, const double* U, const cfs_immersed* ib, double cfl,
+                        double t_end, int32_t* slot) {
+    if (!b || !U || !slot) return CFS_E_ARG;
+    int rc = batch_check_problem(b, cfg, ib, cfl, t_end);
+    if (rc) return rc;
     const bool has_ib = ib != nullptr;
     const int n_ig = has_ib ? ib->n_ig : 0;
-    if (has_ib && (n_ig <= 0 || !ib->image_xy || !ib->coef)) return CFS_E_ARG;
     const ShapeDims sd = shape_dims(b->bc.tile_shape);
     const int tiles_i = (cfg->nx + sd.ti - 1) / sd.ti, tiles_j = (cfg->ny + sd.tj - 1) / sd.tj;
     std::vector<int> index;

@@ -41,6 +41,13 @@ namespace cfs {
 
 constexpr int HALO = 3;            // WENO5 reach: stencil offsets -3..+2
 
+// The state pointers a kernel reads from its StageArgs point to global
+// memory.  The compiler infers that for pointers passed as kernel arguments,
+// not for pointers read from device memory (the batched kernel's per-problem
+// arguments), and then emits flat instructions; G() states it.
+template <typename T> using gptr = __attribute__((address_space(1))) T*;
+template <typename T> __device__ __forceinline__ gptr<T> G(T* p) { return (gptr<T>)p; }
+
 // Tile shape: a workgroup of NT threads owns TI x TJ cells (TI rows along x /
 // i, TJ columns along y / j, j contiguous in memory) and stages SR x SC cells
 // (the tile plus the halo) in LDS.  Its (TI + 1) TJ x-faces and TI (TJ + 1)
@@ -397,7 +404,7 @@ __device__ __forceinline__ long long fetch_cell(const StageArgs& a, int gi, int 
     }
     const long long c = (long long)gi * a.ny + gj;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) v[q] = a.in[q * a.plane + c];
+    for (int q = 0; q < 4; ++q) v[q] = G(a.in)[q * a.plane + c];
     return side >= 0 ? -1 : c;
 }
 
@@ -855,14 +862,14 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
     int unsafe = 0;
 
     // ---- tile + halo -> LDS (corners are never read) ----
-    const bool near_ib = a.tile_ib != nullptr && a.tile_ib[tile];
+    const bool near_ib = a.tile_ib != nullptr && G(a.tile_ib)[tile];
     for (int s = tid; s < SR * SC; s += BLOCK) {
         const int r = s / SC, c = s - r * SC;
         const bool xs = (c >= HALO && c < HALO + TJ), ys = (r >= HALO && r < HALO + TI);
         if (!VISC && !LIMIT && !(xs || ys)) continue;   // viscous cross terms and the limiter read the corners
         double v[4];
         const long long cell = fetch_cell(a, i0 - HALO + r, j0 - HALO + c, v);
-        if (near_ib && cell >= 0 && a.type[cell] == 2) {
+        if (near_ib && cell >= 0 && G(a.type)[cell] == 2) {
             // immersed ghost: its value at the start of this stage, from its donors
             const int k = a.ig_index[cell];
             ghost_value(a.in, a.plane, a.ib_interp, a.ig_donors + 4 * k, a.ig_coef + 10 * k, a.ig_transp + 4 * k,
@@ -959,7 +966,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
         const int i = i0 + il, j = j0 + jl;
         if (i < a.nx && j < a.ny) {
             const long long c = (long long)i * a.ny + j;
-            const int ty = (a.type == nullptr) ? 0 : a.type[c];
+            const int ty = (a.type == nullptr) ? 0 : G(a.type)[c];
             const R rdx = (R)a.dx, rdy = (R)a.dy;
             const R idx = (R)a.inv_dx, idy = (R)a.inv_dy;
             const double c2dt = a.c2 * dt;
@@ -968,7 +975,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
             for (int q = 0; q < 4; ++q) {
                 // the FP64 tile holds the exact state; FP32-bulk re-reads it
                 const double u = (sizeof(R) == 8) ? (double)S[q * NS + (il + HALO) * SC + jl + HALO]
-                                                  : a.in[q * a.plane + c];
+                                                  : G(a.in)[q * a.plane + c];
                 res[q] = u;
                 if (ty == 0) {
                     const R fl = F[q * NF + il * TJ + jl];
@@ -981,8 +988,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
                     const R rhs = tx - tyy;
                     const double L = (double)rhs;
                     if (a.form == 0) res[q] = a.c1 * u + c2dt * L;
-                    else if (a.form == 1) res[q] = (a.c0 * a.u0[q * a.plane + c] + a.c1 * u) + c2dt * L;
-                    else if (a.form == 2) res[q] = (a.c0 * a.u0[q * a.plane + c] + a.c1 * a.u1[q * a.plane + c]) + c2dt * L;
+                    else if (a.form == 1) res[q] = (a.c0 * G(a.u0)[q * a.plane + c] + a.c1 * u) + c2dt * L;
+                    else if (a.form == 2) res[q] = (a.c0 * G(a.u0)[q * a.plane + c] + a.c1 * G(a.u1)[q * a.plane + c]) + c2dt * L;
                     else res[q] = L;
                 } else if (a.form == 3) {
                     res[q] = 0.0;
@@ -991,7 +998,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
             // immersed ghosts were written while staging the tile
             if (ty != 2 || a.form == 3) {
 #pragma unroll
-                for (int q = 0; q < 4; ++q) a.out[q * a.plane + c] = res[q];
+                for (int q = 0; q < 4; ++q) G(a.out)[q * a.plane + c] = res[q];
             }
             if (a.fuse_cfl && ty == 0)
                 smax = cell_speed(res[0], res[1], res[2], res[3], a.gamma, a.dx, a.dy, a.hll_bounds);

@@ -1,5 +1,5 @@
 // serve.cpp -- the continuous-batching request loop (cfs_serve of
-// include/cfs.h), written on the public batch entry points only.
+// include/cfs.h), written on the public batch and single-solve entry points.
 //
 // Rounds: admit what has arrived and fits, advance every resident problem
 // steps_per_poll RK steps, wait, retire what finished.  Admission is in
@@ -7,22 +7,124 @@
 // request does not fit, a later one is admitted only if it leaves at least the
 // oldest one's footprint free, so a large request is not starved by a stream
 // of small ones.
+//
+// Solo lane: a request that fills the GPU on its own (at least as many tiles
+// as the GPU holds stage-kernel workgroups at once) is not batched.  Two
+// worker threads solve such requests in arrival order, each through the
+// single-solve entry points on a stream of its own, while the rounds go on
+// (two, so that the last partial wave of workgroups of one solve's stage
+// overlaps the other's work).  Batching gains nothing for such a request, the single-solve kernel
+// starts each tile with its arguments already in registers (the batched one
+// first reads them from device memory), and the request no longer waits for
+// every other resident problem at each of its steps, which matters because
+// the largest requests take the most steps.  The arithmetic is the same
+// kernel's either way, so its result does not depend on the lane.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
 #include <thread>
 #include <vector>
 
 #include "../../include/cfs.h"
+#include "serve_lane.h"
 
 namespace {
 
 using Clock = std::chrono::steady_clock;
+
+constexpr int kSoloWorkers = 2;
 
 double seconds_since(Clock::time_point t0) {
     return std::chrono::duration<double>(Clock::now() - t0).count();
 }
 
 int64_t footprint_of(const cfs_request& r) { return cfs_batch_footprint(r.cfg, r.ib ? r.ib->n_ig : 0); }
+
+// One request solved alone to its end time; CFS_OK with r.state DONE, or
+// FAILED on a non-finite CFL bound (as a batch slot would fail), else the
+// error of the entry point that failed.
+int32_t solve_alone(cfs_request& r) {
+    cfs_solver* h = nullptr;
+    int32_t rc = cfs_create(r.cfg, nullptr, &h);
+    if (rc) return rc;
+    if (r.ib)
+        rc = cfs_set_immersed(h, r.ib->interp, r.ib->cell_type, r.ib->n_ig, r.ib->ig_cells, r.ib->donors,
+                              r.ib->image_xy, r.ib->coef, r.ib->transp);
+    if (!rc) rc = cfs_set_state(h, r.U);
+    double t = 0.0;
+    int32_t steps = 0;
+    r.state = CFS_SLOT_DONE;
+    while (!rc && t < r.t_end) {
+        int32_t done = 0;
+        const int32_t st = cfs_advance(h, 1 << 30, 0.0, r.cfl, r.t_end, &t, &done);
+        steps += done;
+        if (st == CFS_E_NONFINITE) {
+            r.state = CFS_SLOT_FAILED;
+            break;
+        }
+        rc = st;
+        if (done == 0) break;
+    }
+    if (!rc && r.U_out) rc = cfs_get_state(h, r.U_out);
+    r.t = t;
+    r.steps = steps;
+    cfs_destroy(h);
+    return rc;
+}
+
+struct SoloLane {
+    std::mutex m;
+    std::condition_variable cv;
+    std::deque<int> queue;
+    bool closing = false;
+    int32_t error = CFS_OK;      // first failure of an entry point (under m)
+    std::atomic<int> finished{0};
+
+    void run(const cfs_batch* b, cfs_request* reqs, Clock::time_point t0) {
+        const int32_t dev_rc = batch_make_device_current(b);
+        for (;;) {
+            int k;
+            {
+                std::unique_lock<std::mutex> lk(m);
+                cv.wait(lk, [&] { return closing || !queue.empty(); });
+                if (queue.empty()) return;
+                k = queue.front();
+                queue.pop_front();
+            }
+            cfs_request& r = reqs[k];
+            r.admitted_s = seconds_since(t0);
+            const int32_t rc = dev_rc ? dev_rc : solve_alone(r);
+            r.finished_s = seconds_since(t0);
+            if (rc) {
+                std::lock_guard<std::mutex> lk(m);
+                if (!error) error = rc;
+            }
+            finished.fetch_add(1);
+        }
+    }
+    void push(int k) {
+        {
+            std::lock_guard<std::mutex> lk(m);
+            queue.push_back(k);
+        }
+        cv.notify_one();
+    }
+    void close() {   // the workers end after the solves in hand (nothing is queued on success)
+        {
+            std::lock_guard<std::mutex> lk(m);
+            closing = true;
+            queue.clear();
+        }
+        cv.notify_all();
+    }
+    int32_t failure() {
+        std::lock_guard<std::mutex> lk(m);
+        return error;
+    }
+};
 
 }  // namespace
 
@@ -32,13 +134,17 @@ extern "C" int32_t cfs_serve(cfs_batch* b, cfs_request* reqs, int32_t n, int32_t
     int32_t nslots = 0;
     int rc = cfs_batch_capacity(b, &arena, &nslots);
     if (rc) return rc;
+    const int64_t resident_wg = batch_resident_workgroups(b);
     std::vector<int64_t> fp((size_t)n);
+    std::vector<char> solo((size_t)n, 0);
     for (int k = 0; k < n; ++k) {
         cfs_request& r = reqs[k];
         if (!r.cfg || !r.U || !(r.t_end > 0.0) || !(r.arrival_s >= 0.0)) return CFS_E_ARG;
+        if ((rc = batch_check_problem(b, r.cfg, r.ib, r.cfl, r.t_end))) return rc;
         fp[k] = footprint_of(r);
         if (fp[k] < 0) return CFS_E_ARG;
-        if (fp[k] > arena) return CFS_E_FULL;
+        solo[k] = batch_problem_tiles(b, r.cfg) >= resident_wg;
+        if (!solo[k] && fp[k] > arena) return CFS_E_FULL;
         r.admitted_s = r.finished_s = r.t = 0.0;
         r.steps = 0;
         r.state = CFS_SLOT_FREE;
@@ -53,15 +159,35 @@ extern "C" int32_t cfs_serve(cfs_batch* b, cfs_request* reqs, int32_t n, int32_t
     size_t next = 0;
     int resident = 0, finished = 0;
     const Clock::time_point t0 = Clock::now();
-    while (finished < n) {
+    SoloLane lane;
+    std::vector<std::thread> workers;
+    auto stop = [&](int32_t code) {
+        lane.close();
+        for (auto& w : workers) w.join();
+        return code;
+    };
+    try {
+        for (int w = 0; w < kSoloWorkers; ++w) workers.emplace_back([&] { lane.run(b, reqs, t0); });
+    } catch (...) {
+        return stop(CFS_E_OOM);
+    }
+    while (finished + lane.finished.load() < n) {
         const double now = seconds_since(t0);
-        while (next < order.size() && reqs[order[next]].arrival_s <= now) waiting.push_back(order[next++]);
+        while (next < order.size() && reqs[order[next]].arrival_s <= now) {
+            const int k = order[next++];
+            if (solo[k]) {
+                lane.push(k);
+            } else {
+                waiting.push_back(k);
+            }
+        }
+        if ((rc = lane.failure())) return stop(rc);
         // admission
         int64_t reserve = 0;
         for (size_t w = 0; w < waiting.size();) {
             const int k = waiting[w];
             int64_t free_bytes = 0;
-            if ((rc = cfs_batch_free_bytes(b, &free_bytes))) return rc;
+            if ((rc = cfs_batch_free_bytes(b, &free_bytes))) return stop(rc);
             int32_t slot = -1;
             rc = CFS_E_FULL;
             if (fp[k] + reserve <= free_bytes)
@@ -75,20 +201,20 @@ extern "C" int32_t cfs_serve(cfs_batch* b, cfs_request* reqs, int32_t n, int32_t
                 if (w == 0) reserve = fp[k];   // the oldest waiting request keeps its room
                 ++w;
             } else {
-                return rc;
+                return stop(rc);
             }
         }
         if (resident == 0) {
-            if (next < order.size()) {
-                const double wait = reqs[order[next]].arrival_s - seconds_since(t0);
-                if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
-            }
+            // nothing to batch: wait for the next arrival or the solo lane
+            double wait = 1e-3;
+            if (next < order.size()) wait = std::min(wait, reqs[order[next]].arrival_s - seconds_since(t0));
+            if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
             continue;
         }
         // one round of continuous batching
-        if ((rc = cfs_batch_step(b, steps_per_poll))) return rc;
+        if ((rc = cfs_batch_step(b, steps_per_poll))) return stop(rc);
         int32_t m = 0;
-        if ((rc = cfs_batch_poll(b, info.data(), nslots, &m))) return rc;
+        if ((rc = cfs_batch_poll(b, info.data(), nslots, &m))) return stop(rc);
         const double done_at = seconds_since(t0);
         for (int q = 0; q < m && q < nslots; ++q) {
             const cfs_batch_info& in = info[q];
@@ -100,11 +226,11 @@ extern "C" int32_t cfs_serve(cfs_batch* b, cfs_request* reqs, int32_t n, int32_t
             r.t = in.t;
             r.steps = in.steps;
             r.state = in.state;
-            if ((rc = cfs_batch_retire(b, in.slot, r.U_out))) return rc;
+            if ((rc = cfs_batch_retire(b, in.slot, r.U_out))) return stop(rc);
             slot_req[in.slot] = -1;
             --resident;
             ++finished;
         }
     }
-    return CFS_OK;
+    return stop(lane.failure());
 }

@@ -349,9 +349,16 @@ typedef struct cfs_request {
  * and a slot is free (a request that does not fit waits; later ones that fit
  * may pass it), every resident problem advances steps_per_poll RK steps per
  * round, and finished problems are retired at the end of the round they
- * finish in, freeing their space for waiting requests.  Returns when every
- * request has finished; requests whose footprint exceeds the whole arena are
- * CFS_E_FULL before anything runs.  No reference counterpart. */
+ * finish in, freeing their space for waiting requests.  A request that fills
+ * the GPU on its own (at least as many tiles as the GPU holds stage-kernel
+ * workgroups at once) is solved instead by worker threads through the
+ * single-solve entry points (cfs_create, cfs_advance; its own stream and
+ * device memory), up to two such requests at a time in arrival order,
+ * alongside the rounds; its result is the same as in the batch.  Returns when every request
+ * has finished; a request the batch would refuse is CFS_E_ARG (or
+ * CFS_E_UNSUPPORTED), and one whose footprint exceeds the whole arena without
+ * filling the GPU is CFS_E_FULL, before anything runs.  No reference
+ * counterpart. */
 int32_t cfs_serve(cfs_batch* b, cfs_request* reqs, int32_t n, int32_t steps_per_poll);
 
 #ifdef __cplusplus

@@ -59,3 +59,30 @@ def test_serve_rejects_a_request_larger_than_the_arena():
         with pytest.raises(cfs.CfsError) as e:
             b.serve([dict(cfg=cfg, U=U, ib=None, cfl=0.5, t_end=0.01, arrival_s=0.0)])
         assert e.value.status == cfs.CFS_E_FULL
+
+
+@pytest.mark.parametrize("precision", [cfs.PREC_FP64, cfs.PREC_FP32_BULK])
+def test_serve_solo_lane_for_requests_that_fill_the_gpu(precision):
+    # a 1024^2 request (4,416 tiles: more than the GPU holds workgroups at
+    # once) goes to the solo lane: it is served although its footprint is
+    # larger than the whole arena, and its result equals the solve alone, as
+    # do those of the small requests batched meanwhile
+    reqs = cfs.serving_requests(9, 6, rate=400.0, precision=precision, riemann_sizes=(40, 64),
+                                cylinder_ny=(32, 48), t_end_riemann=(0.01, 0.02), t_end_cylinder=(0.05, 0.08))
+    big_cfg, big_U = cfs.riemann_batch_problems(17, count=1, sizes=(1024,), precision=precision, rk=2)[0]
+    reqs.insert(1, dict(cfg=big_cfg, U=big_U, ib=None, cfl=0.5, t_end=0.002, arrival_s=0.0))
+    fps = [cfs.Batch.footprint(r["cfg"], r["ib"]["n_ig"] if r["ib"] is not None else 0) for r in reqs]
+    arena = 3 * max(fps[:1] + fps[2:])
+    assert fps[1] > arena
+    with cfs.Batch(rk=2, precision=precision, max_problems=8, arena_bytes=arena, limiter=True) as b:
+        res = b.serve(reqs, steps_per_poll=3)
+    for k, (rq, r) in enumerate(zip(reqs, res)):
+        ref, t, done, ok = _alone(rq)
+        print("request %d %dx%d: %s steps %d, latency %.4f s"
+              % (k, rq["cfg"].nx, rq["cfg"].ny, r["state"], r["steps"], r["latency_s"]))
+        if not ok:
+            assert r["state"] == cfs.SLOT_FAILED
+            continue
+        assert r["state"] == cfs.SLOT_DONE and r["steps"] == done and r["t"] == t
+        assert np.array_equal(r["U"].view(np.uint64), ref.view(np.uint64)), "request %d differs" % k
+    assert res[1]["state"] == cfs.SLOT_DONE and res[1]["steps"] > 0
