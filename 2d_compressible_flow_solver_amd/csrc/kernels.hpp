@@ -229,30 +229,33 @@ __device__ __forceinline__ bool fast_div_safe(double v) {
 // (source/interpolation/Weno5th.cpp:4-28: Jiang-Shu smoothness indicators,
 // eps = 1e-2, ideal weights 0.1/0.6/0.3).  Operation order as the reference.
 //
-// FP32-bulk evaluates the same formula with fused multiply-adds and one
-// reciprocal of the weight sum (the value is a convex combination either way;
-// FP32 results are compared with FP64 within a tolerance, not bit for bit):
-// (a0 q0 + a1 q1 + a2 q2) / (a0 + a1 + a2), a_k = d_k / (eps + beta_k)^2.
+// FP32-bulk evaluates the same function rearranged for fused multiply-adds
+// (FP32 results are compared with FP64 within a tolerance, not bit for bit):
+// value = (sum_k r_k D_k) / (sum_k d_k r_k), r_k = 1 / (eps' + beta'_k)^2,
+// with beta'_k = (12/13) beta_k and eps' = (12/13) eps (the common factor
+// (13/12)^2 of the r_k cancels) and D_k = d_k q_k, the ideal weights folded
+// into the candidate stencils' coefficients.
 __device__ __forceinline__ float weno5_f32(float v0, float v1, float v2, float v3, float v4) {
-    using C = K<float>;
+    constexpr float k = 3.0f / 13.0f, e = 1e-2f * 12.0f / 13.0f;   // (12/13)(1/4) and (12/13) eps
     const float s0 = __builtin_fmaf(-2.0f, v1, v0) + v2;
     const float t0 = __builtin_fmaf(3.0f, v2, __builtin_fmaf(-4.0f, v1, v0));
-    const float b0 = __builtin_fmaf(C::c1312 * s0, s0, (0.25f * t0) * t0);
+    const float b0 = __builtin_fmaf(s0, s0, (k * t0) * t0);
     const float s1 = __builtin_fmaf(-2.0f, v2, v1) + v3;
     const float t1 = v1 - v3;
-    const float b1 = __builtin_fmaf(C::c1312 * s1, s1, (0.25f * t1) * t1);
+    const float b1 = __builtin_fmaf(s1, s1, (k * t1) * t1);
     const float s2 = __builtin_fmaf(-2.0f, v3, v2) + v4;
     const float t2 = __builtin_fmaf(-4.0f, v3, __builtin_fmaf(3.0f, v2, v4));
-    const float b2 = __builtin_fmaf(C::c1312 * s2, s2, (0.25f * t2) * t2);
-    const float e0 = C::eps + b0, e1 = C::eps + b1, e2 = C::eps + b2;
-    const float a0 = C::d0 * __builtin_amdgcn_rcpf(e0 * e0);
-    const float a1 = C::d1 * __builtin_amdgcn_rcpf(e1 * e1);
-    const float a2 = C::d2 * __builtin_amdgcn_rcpf(e2 * e2);
-    const float q0 = __builtin_fmaf(11.0f / 6.0f, v2, __builtin_fmaf(-7.0f / 6.0f, v1, v0 * (1.0f / 3.0f)));
-    const float q1 = __builtin_fmaf(1.0f / 3.0f, v3, __builtin_fmaf(5.0f / 6.0f, v2, v1 * (-1.0f / 6.0f)));
-    const float q2 = __builtin_fmaf(-1.0f / 6.0f, v4, __builtin_fmaf(5.0f / 6.0f, v3, v2 * (1.0f / 3.0f)));
-    const float num = __builtin_fmaf(a2, q2, __builtin_fmaf(a1, q1, a0 * q0));
-    return num * __builtin_amdgcn_rcpf((a0 + a1) + a2);
+    const float b2 = __builtin_fmaf(s2, s2, (k * t2) * t2);
+    const float e0 = e + b0, e1 = e + b1, e2 = e + b2;
+    const float r0 = __builtin_amdgcn_rcpf(e0 * e0);
+    const float r1 = __builtin_amdgcn_rcpf(e1 * e1);
+    const float r2 = __builtin_amdgcn_rcpf(e2 * e2);
+    const float D0 = __builtin_fmaf(0.1f * 11.0f / 6.0f, v2, __builtin_fmaf(-0.1f * 7.0f / 6.0f, v1, v0 * (0.1f / 3.0f)));
+    const float D1 = __builtin_fmaf(0.6f / 3.0f, v3, __builtin_fmaf(0.6f * 5.0f / 6.0f, v2, v1 * (-0.6f / 6.0f)));
+    const float D2 = __builtin_fmaf(-0.3f / 6.0f, v4, __builtin_fmaf(0.3f * 5.0f / 6.0f, v3, v2 * (0.3f / 3.0f)));
+    const float num = __builtin_fmaf(r2, D2, __builtin_fmaf(r1, D1, r0 * D0));
+    const float den = __builtin_fmaf(0.3f, r2, __builtin_fmaf(0.6f, r1, 0.1f * r0));
+    return num * __builtin_amdgcn_rcpf(den);
 }
 
 template <typename R, bool FAST>
