@@ -418,3 +418,16 @@ def test_limiter_fp32_bulk_l1_density():
     l1 = np.abs(s.get_state()[0] - o.get_state()[0]).mean()
     print("limiter FP32-bulk L1(rho) after 100 steps:", l1)
     assert l1 <= 1e-4
+
+
+def test_laxliu_anchor_fixture():
+    # tests/golden/laxliu3_32x32_rk3_dt1e-3_20steps.json: the committed
+    # bit pattern of 20 RK3 steps, reproduced through the C-ABI
+    import test_golden as TG
+    g, cfg, U0 = TG.laxliu_anchor()
+    s = cfs.Solver(cfg)
+    s.set_state(U0)
+    s.advance(g["steps"], dt=g["dt"])
+    U = s.get_state()
+    s.close()
+    assert TG.digest(U) == g["sha256"]
