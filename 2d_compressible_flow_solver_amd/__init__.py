@@ -259,6 +259,8 @@ def _lib():
         getattr(lib, name).argtypes = [vp, vp]
     lib.cfs_circle_setup.argtypes = [P(Config), dbl, dbl, dbl, i32, i32, i32, i32, vp, P(i32), vp, vp, vp, vp,
                                      vp]
+    lib.cfs_circle_setup_device.argtypes = [P(Config), dbl, dbl, dbl, i32, i32, i32, i32, vp, P(i32), vp, vp, vp,
+                                            vp, vp, P(i32)]
     lib.cfs_set_immersed.argtypes = [vp, i32, vp, i32, vp, vp, vp, vp, vp]
     lib.cfs_stage.argtypes = [vp, i32, dbl]
     lib.cfs_step.argtypes = [vp, dbl]
@@ -294,7 +296,8 @@ def _lib():
                  "cfs_batch_free_bytes", "cfs_batch_capacity", "cfs_serve"):
         getattr(lib, name).restype = i32
     for name in ("cfs_create", "cfs_set_state", "cfs_get_state", "cfs_set_state_device",
-                 "cfs_get_state_device", "cfs_circle_setup", "cfs_set_immersed", "cfs_stage",
+                 "cfs_get_state_device", "cfs_circle_setup", "cfs_circle_setup_device", "cfs_set_immersed",
+                 "cfs_stage",
                  "cfs_step", "cfs_advance", "cfs_compute_dt", "cfs_checksums", "cfs_last_kernel_ms"):
         getattr(lib, name).restype = i32
     _LIB = lib
@@ -330,8 +333,10 @@ def tile_shape_info(shape):
     return ti.value, tj.value, nt.value
 
 
-def circle_setup(cfg, cx, cy, radius, n_lagrangian=400, n_layers=3, interp=IB_BILINEAR_POSITIVE):
-    """Immersed-boundary tables of a circle (host code of libcfs.so; no GPU needed).
+def circle_setup(cfg, cx, cy, radius, n_lagrangian=400, n_layers=3, interp=IB_BILINEAR_POSITIVE, device=False):
+    """Immersed-boundary tables of a circle.  device=False: host code of
+    libcfs.so (no GPU needed); device=True: cfs_circle_setup_device (the same
+    tables, computed on the GPU; out["n_host"] = ghosts finished on the host).
     interp: IB_REFERENCE (the reference's LDLT-on-lower-triangle interpolation,
     as written) or IB_BILINEAR_POSITIVE (exact bilinear fit when non-degenerate
     and non-negative, else inverse-distance weights).
@@ -340,24 +345,47 @@ def circle_setup(cfg, cx, cy, radius, n_lagrangian=400, n_layers=3, interp=IB_BI
     n = cfg.nx * cfg.ny
     cell_type = np.zeros(n, dtype=np.uint8)
     n_ig = ctypes.c_int32(0)
-    z = np.zeros(1, dtype=np.int32)
-    rc = lib.cfs_circle_setup(ctypes.byref(cfg), cx, cy, radius, n_lagrangian, n_layers, interp, 0,
-                              _ptr(cell_type), ctypes.byref(n_ig), None, None, None, None, None)
-    if rc not in (CFS_OK, CFS_E_ARG):
-        _check(rc, "cfs_circle_setup")
-    cap = max(n_ig.value, 1)
-    out = dict(interp=interp, cell_type=cell_type,
-               ig_cells=np.zeros(cap, dtype=np.int32), donors=np.zeros(4 * cap, dtype=np.int32),
-               image_xy=np.zeros(2 * cap), coef=np.zeros(10 * cap), transp=np.zeros(4 * cap, dtype=np.int32))
-    rc = lib.cfs_circle_setup(ctypes.byref(cfg), cx, cy, radius, n_lagrangian, n_layers, interp, cap,
-                              _ptr(cell_type), ctypes.byref(n_ig), _ptr(out["ig_cells"]), _ptr(out["donors"]),
-                              _ptr(out["image_xy"]), _ptr(out["coef"]), _ptr(out["transp"]))
-    _check(rc, "cfs_circle_setup")
+    n_host = ctypes.c_int32(0)
+    name = "cfs_circle_setup_device" if device else "cfs_circle_setup"
+    fn = getattr(lib, name)
+    extra = (ctypes.byref(n_host),) if device else ()
+
+    def tables(cap):
+        return dict(interp=interp, cell_type=cell_type,
+                    ig_cells=np.zeros(cap, dtype=np.int32), donors=np.zeros(4 * cap, dtype=np.int32),
+                    image_xy=np.zeros(2 * cap), coef=np.zeros(10 * cap), transp=np.zeros(4 * cap, dtype=np.int32))
+
+    def call(cap, out):
+        args = (None,) * 5 if out is None else tuple(_ptr(out[k]) for k in
+                                                      ("ig_cells", "donors", "image_xy", "coef", "transp"))
+        return fn(ctypes.byref(cfg), cx, cy, radius, n_lagrangian, n_layers, interp, cap, _ptr(cell_type),
+                  ctypes.byref(n_ig), *args, *extra)
+
+    if device:
+        # one call with room for the ghosts a ring of n_layers + 2 cells
+        # around the body can hold (a second only if that was short)
+        h = min(cfg.lx / cfg.nx, cfg.ly / cfg.ny)
+        cap = int(2.0 * np.pi * radius * (n_layers + 2) / h * 1.5) + 64
+        out = tables(cap)
+        rc = call(cap, out)
+        if rc == CFS_E_ARG and n_ig.value > cap:
+            cap = n_ig.value
+            out = tables(cap)
+            rc = call(cap, out)
+    else:
+        rc = call(0, None)   # count first
+        if rc not in (CFS_OK, CFS_E_ARG):
+            _check(rc, name)
+        cap = max(n_ig.value, 1)
+        out = tables(cap)
+        rc = call(cap, out)
+    _check(rc, name)
     k = n_ig.value
     out["n_ig"] = k
+    if device:
+        out["n_host"] = n_host.value
     for key, w in (("ig_cells", 1), ("donors", 4), ("image_xy", 2), ("coef", 10), ("transp", 4)):
         out[key] = out[key][: w * k]
-    del z
     return out
 
 

@@ -43,7 +43,7 @@ def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     objs = []
     jobs = []
-    for src in ["cfs_api.hip", "batch.hip"]:   # compiled in parallel
+    for src in ["cfs_api.hip", "batch.hip", "ib_setup.hip"]:   # compiled in parallel
         obj = os.path.join(BUILD, src + ".o")
         cmd = [HIPCC, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj] + COMMON + DEVICE
         if verbose:
@@ -54,7 +54,7 @@ def build(verbose=False):
         out, _ = pr.communicate()
         if pr.returncode != 0:
             raise RuntimeError("build failed: %s\n%s" % (" ".join(cmd), out))
-    for src in ["ib_setup.cpp", "serve.cpp"]:
+    for src in ["serve.cpp"]:
         obj = os.path.join(BUILD, src + ".o")
         _run(["g++", "-c", os.path.join(CSRC, src), "-o", obj] + COMMON, verbose)
         objs.append(obj)

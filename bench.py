@@ -34,6 +34,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
        python bench.py --workload batch | serve    (batched / serving measurements)
        python bench.py --workload kh               (viscous shear layer 4096^2 to t = 1, FP64 and FP32-bulk)
        python bench.py --workload riemann          (Lax-Liu 400^2 to t = 0.3, GPU vs CPU restatement, bitwise)
+       python bench.py --workload ibsetup          (immersed-boundary set-up latency, host vs device)
 """
 import argparse
 import importlib
@@ -543,6 +544,38 @@ def main_serve(args):
     print(json.dumps(out))
 
 
+def main_ibsetup(args):
+    """Immersed-boundary set-up latency, host (cfs_circle_setup) against device
+    (cfs_circle_setup_device), median of 15 calls after a warm-up call, on
+    the headline body and on served cylinder sizes; the tables are compared
+    bit for bit on every geometry."""
+    cases = [("cylinder_2048x1024_r0.125", cfs.config_cylinder(2048, 1024, 4.0, 2.0), 0.125),
+             ("cylinder_1024x512_r0.1", cfs.config_cylinder(1024, 512, 4.0, 2.0), 0.1),
+             ("cylinder_512x256_r0.2", cfs.config_cylinder(512, 256, 4.0, 2.0), 0.2),
+             ("cylinder_4096x2048_r0.125", cfs.config_cylinder(4096, 2048, 4.0, 2.0), 0.125)]
+    rows = []
+    for name, cfg, r in cases:
+        cfs.circle_setup(cfg, 1.0, 1.0, r, device=True)   # warm-up: module load, first launches
+        th, td = [], []
+        for _ in range(15):
+            t0 = time.perf_counter()
+            h = cfs.circle_setup(cfg, 1.0, 1.0, r)
+            t1 = time.perf_counter()
+            d = cfs.circle_setup(cfg, 1.0, 1.0, r, device=True)
+            t2 = time.perf_counter()
+            th.append(t1 - t0)
+            td.append(t2 - t1)
+        same = all(np.array_equal(np.asarray(h[k]).view(np.uint8), np.asarray(d[k]).view(np.uint8))
+                   for k in ("cell_type", "ig_cells", "donors", "image_xy", "coef", "transp"))
+        rows.append({"workload": name, "ghosts": d["n_ig"], "finished_on_host": d["n_host"],
+                     "host_ms": round(1e3 * float(np.median(th)), 3), "device_ms": round(1e3 * float(np.median(td)), 3),
+                     "identical": bool(same)})
+        print("ibsetup: %s" % json.dumps(rows[-1]), file=sys.stderr, flush=True)
+    print(json.dumps({"metric": "immersed-boundary set-up latency (ms, median of 15)", "unit": "ms",
+                      "higher_is_better": False, "n_gpus": 1, "data": "circle bodies (no randomness)",
+                      "rows": rows}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -552,7 +585,8 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-bulk leg")
     ap.add_argument("--no-reference-scheme", action="store_true", help="skip the no-limiter leg")
     ap.add_argument("--fp32-only", action="store_true", help="time the FP32-bulk leg only (profiling)")
-    ap.add_argument("--workload", choices=["cylinder", "batch", "serve", "kh", "riemann"], default="cylinder",
+    ap.add_argument("--workload", choices=["cylinder", "batch", "serve", "kh", "riemann", "ibsetup"],
+                    default="cylinder",
                     help="cylinder: the headline line; batch / serve: the batched-solves and serving measurements")
     ap.add_argument("--kh-n", type=int, default=4096, help="shear-layer grid side (--workload kh)")
     ap.add_argument("--kh-t-end", type=float, default=1.0, help="shear-layer end time (--workload kh)")
@@ -589,6 +623,9 @@ def main():
         return
     if args.workload == "riemann":
         main_riemann(args)
+        return
+    if args.workload == "ibsetup":
+        main_ibsetup(args)
         return
     if args.copy_calibration:
         cfg, ib, U = setup(cfs.PREC_FP64)

@@ -153,6 +153,21 @@ int32_t cfs_circle_setup(const cfs_config* cfg, double cx, double cy, double rad
                          int32_t* donors, double* image_xy, double* coef,
                          int32_t* transp);
 
+/* cfs_circle_setup on the GPU (the device current on the calling thread):
+ * the same outputs, bit for bit.  Kernels classify the cells of the body's
+ * bounding box, mark the ghost layers, list the ghosts in i-major order and
+ * compute each ghost's image point, donors and coefficients; a ghost whose
+ * five nearest donor candidates include two at equal distance (where the
+ * reference's std::sort decides their order) is finished on the host with that
+ * sort, and *n_host (optional) receives how many were.  Same errors as
+ * cfs_circle_setup, and CFS_E_HIP.  Reference: Circle.cpp:12-215 (the host
+ * set-up it replaces at admission time). */
+int32_t cfs_circle_setup_device(const cfs_config* cfg, double cx, double cy, double radius,
+                                int32_t n_lagrangian, int32_t n_layers, int32_t interp, int32_t cap,
+                                uint8_t* cell_type, int32_t* n_ig, int32_t* ig_cells,
+                                int32_t* donors, double* image_xy, double* coef,
+                                int32_t* transp, int32_t* n_host);
+
 /* Uploads the immersed-boundary tables produced by cfs_circle_setup.  From
  * then on every stage first fills the immersed-ghost cells from their image
  * points (Run.cpp:156-162 -> Cell::comImmersedGhostPhase, Cell.cpp:172-181)
