@@ -431,3 +431,50 @@ def test_laxliu_anchor_fixture():
     U = s.get_state()
     s.close()
     assert TG.digest(U) == g["sha256"]
+
+
+def test_cylinder_mach3_full_size_cfl_steps():
+    # the headline workload at its full 2048 x 1024 size: Mach-3 cylinder with
+    # the positivity limiter and the bilinear immersed-boundary tables, 3 CFL
+    # RK3 steps on the device against the oracle (its multi-rank form, 16
+    # threads, the same CFL time steps), bit for bit
+    fs = (1.4, 3.0, 0.0, 1.0)
+    cfg = cfs.config_cylinder(2048, 1024, 4.0, 2.0, freestream=fs, positivity=2, hll_bounds=0)
+    ib = cfs.circle_setup(cfg, 1.0, 1.0, 0.125, 400, 3, interp=cfs.IB_BILINEAR_POSITIVE)
+    U = cfs.cylinder_freestream(cfg, ib["cell_type"], freestream=fs)
+    s = cfs.Solver(cfg)
+    s.set_immersed(ib)
+    s.set_state(U)
+    t, done = s.advance(3, dt=0.0, cfl=0.5)
+    o = O.Oracle(cfg, nthreads=16)
+    o.set_ranks(16)
+    o.set_ib_table(ib)
+    o.set_state(U)
+    tc = 0.0
+    for _ in range(3):
+        dt = 0.5 / o.cfl_smax()
+        o.step(dt)
+        tc += dt
+    assert done == 3 and t == tc
+    _assert_parity(s.get_state(), o.get_state(), "cylinder 2048x1024 3 CFL steps")
+    s.close()
+    o.close()
+
+
+def test_shear_layer_full_size_cfl_step():
+    # the viscous shear layer at its full 4096^2 size: one CFL RK3 step (three
+    # stages of the viscous kernel) against the oracle, bit for bit
+    cfg = cfs.config_kelvin_helmholtz(4096)
+    U = cfs.kelvin_helmholtz(cfg)
+    s = cfs.Solver(cfg)
+    s.set_state(U)
+    t, done = s.advance(1, dt=0.0, cfl=0.5)
+    o = O.Oracle(cfg, nthreads=16)
+    o.set_ranks(16)
+    o.set_state(U)
+    dt = 0.5 / o.cfl_smax()
+    o.step(dt)
+    assert done == 1 and t == dt
+    _assert_parity(s.get_state(), o.get_state(), "shear layer 4096^2 1 CFL step")
+    s.close()
+    o.close()
