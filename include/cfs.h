@@ -71,7 +71,8 @@ enum { CFS_IB_REFERENCE = 0, CFS_IB_BILINEAR_POSITIVE = 1 };
 enum { CFS_CELL_FLUID = 0, CFS_CELL_SOLID = 1, CFS_CELL_IMMERSED_GHOST = 2 };
 
 typedef struct cfs_config {
-    int32_t nx, ny;          /* cells along x and y (CartesianMesh::setnumberCells*) */
+    int32_t nx, ny;          /* cells along x and y (CartesianMesh::setnumberCells*); at least 3 each,
+                                nx*ny < 2^31 (a periodic wrap of the 3-cell halo stays in one period) */
     double lx, ly;           /* domain lengths; dx = lx/nx (CartesianMesh.cpp:150-153) */
     double x0, y0;           /* domain origin; centre of (i,j) = ((i+0.5)*dx + x0, (j+0.5)*dy + y0) */
     double gamma;            /* ratio of specific heats (param.h:5) */
