@@ -704,7 +704,7 @@ template <> struct VT<float> {
 // gradient arrays at cell cs[1 + m]
 template <typename R, typename Sh>
 __device__ __forceinline__ void viscous_flux(const ViscLDS<R, Sh>& V, const int cs[6], const int gs[4], bool xface,
-                                             R h, R mu, R kappa, R out[3]) {
+                                             R h, bool h_pow2, R inv_h, R mu, R kappa, R out[3]) {
     const R* un = xface ? V.pu : V.pv;
     const R* ut = xface ? V.pv : V.pu;
     const R* gn = xface ? V.gyu : V.gxv;   // tangential derivative of the normal velocity
@@ -719,9 +719,10 @@ __device__ __forceinline__ void viscous_flux(const ViscLDS<R, Sh>& V, const int 
             dn_ut += VT<R>::thr[r][k] * ut[cs[k]];
             dn_T += VT<R>::thr[r][k] * V.pT[cs[k]];
         }
-        dn_un = K<R>::div(dn_un, h);
-        dn_ut = K<R>::div(dn_ut, h);
-        dn_T = K<R>::div(dn_T, h);
+        // x/h == x*(1/h) bit for bit when h is a power of two
+        dn_un = h_pow2 ? dn_un * inv_h : K<R>::div(dn_un, h);
+        dn_ut = h_pow2 ? dn_ut * inv_h : K<R>::div(dn_ut, h);
+        dn_T = h_pow2 ? dn_T * inv_h : K<R>::div(dn_T, h);
         R dt_un = R(0), dt_ut = R(0), unf = R(0), utf = R(0);
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
@@ -798,7 +799,8 @@ __device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, d
             for (int k = 0; k < 6; ++k) cs[k] = cell[k];
             const bool xface = s < NXF;
             R fv[3];
-            viscous_flux<R, Sh>(*V, cs, gcell, xface, xface ? (R)a.dx : (R)a.dy, (R)a.mu, (R)a.kappa, fv);
+            viscous_flux<R, Sh>(*V, cs, gcell, xface, xface ? (R)a.dx : (R)a.dy, xface ? a.dx_pow2 != 0 : a.dy_pow2 != 0,
+                                xface ? (R)a.inv_dx : (R)a.inv_dy, (R)a.mu, (R)a.kappa, fv);
             fl[1] = fl[1] - fv[0];
             fl[2] = fl[2] - fv[1];
             fl[3] = fl[3] - fv[2];
@@ -936,8 +938,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
                     du += VT<R>::thc[k] * V->pu[s - 2 + k];
                     dv += VT<R>::thc[k] * V->pv[s - 2 + k];
                 }
-                V->gyu[q] = K<R>::div(du, rdy);
-                V->gyv[q] = K<R>::div(dv, rdy);
+                V->gyu[q] = a.dy_pow2 ? du * (R)a.inv_dy : K<R>::div(du, rdy);
+                V->gyv[q] = a.dy_pow2 ? dv * (R)a.inv_dy : K<R>::div(dv, rdy);
             } else {
                 const int t = q - VL::NGY;
                 const int gr = t / VL::GXW, gc = t - gr * VL::GXW;
@@ -947,8 +949,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
                     du += VT<R>::thc[k] * V->pu[s + (k - 2) * SC];
                     dv += VT<R>::thc[k] * V->pv[s + (k - 2) * SC];
                 }
-                V->gxu[t] = K<R>::div(du, rdx);
-                V->gxv[t] = K<R>::div(dv, rdx);
+                V->gxu[t] = a.dx_pow2 ? du * (R)a.inv_dx : K<R>::div(du, rdx);
+                V->gxv[t] = a.dx_pow2 ? dv * (R)a.inv_dx : K<R>::div(dv, rdx);
             }
         }
         __syncthreads();
@@ -1014,10 +1016,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
 // __launch_bounds__ is the minimum number of waves per execution unit (SIMD)
 // on this target, which sets the VGPR budget (512 / waves).  The wanted
 // figure -- FP64 5, FP32 6, with the limiter FP64 4 (105 VGPRs), viscous FP64
-// 3 (133 VGPRs) and FP32 5 -- is capped by the waves per SIMD that the
+// 3 (147 VGPRs) and FP32 6 -- is capped by the waves per SIMD that the
 // workgroup's LDS (160 KB per CU) allows.
 template <typename R, bool VISC, bool LIMIT, typename Sh> struct Occupancy {
-    static constexpr int want = VISC ? (sizeof(R) == 8 ? 3 : 5) : LIMIT ? (sizeof(R) == 8 ? 4 : 6) : (sizeof(R) == 8 ? 5 : 6);
+    static constexpr int want = VISC ? (sizeof(R) == 8 ? 3 : 6) : LIMIT ? (sizeof(R) == 8 ? 4 : 6) : (sizeof(R) == 8 ? 5 : 6);
     static constexpr int extra = VISC ? (int)sizeof(ViscLDS<R, Sh>) : LIMIT ? (int)sizeof(LimitLDS<R, Sh>) : 0;
     static constexpr int lds = (int)sizeof(R) * (Sh::SR * Sh::SC * 4 + (Sh::NXF + Sh::NYF) * 4) + 8 * (Sh::NT / 64) + extra;
     static constexpr int by_lds = (160 * 1024 / lds) * (Sh::NT / 64) / 4;
