@@ -868,12 +868,22 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
 
     // ---- tile + halo -> LDS (corners are never read) ----
     const bool near_ib = a.tile_ib != nullptr && G(a.tile_ib)[tile];
+    // a tile whose halo lies inside the grid reads its cells directly: no
+    // boundary ghost, no periodic wrap (the loads fetch_cell makes there)
+    const bool interior = i0 >= HALO && i0 + TI + HALO <= a.nx && j0 >= HALO && j0 + TJ + HALO <= a.ny;
     for (int s = tid; s < SR * SC; s += BLOCK) {
         const int r = s / SC, c = s - r * SC;
         const bool xs = (c >= HALO && c < HALO + TJ), ys = (r >= HALO && r < HALO + TI);
         if (!VISC && !LIMIT && !(xs || ys)) continue;   // viscous cross terms and the limiter read the corners
         double v[4];
-        const long long cell = fetch_cell(a, i0 - HALO + r, j0 - HALO + c, v);
+        long long cell;
+        if (interior) {
+            cell = (long long)(i0 - HALO + r) * a.ny + (j0 - HALO + c);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[q] = G(a.in)[q * a.plane + cell];
+        } else {
+            cell = fetch_cell(a, i0 - HALO + r, j0 - HALO + c, v);
+        }
         if (near_ib && cell >= 0 && G(a.type)[cell] == 2) {
             // immersed ghost: its value at the start of this stage, from its donors
             const int k = a.ig_index[cell];
