@@ -103,6 +103,47 @@ def test_weno_fifth_order_on_smooth_data():
     assert all(4.8 < o < 5.5 for o in orders), (errs, orders)
 
 
+
+def _weno5_source_order(v):
+    """Weno5th.cpp:4-28 evaluated in Python floats (IEEE double, round to
+    nearest, no fused operations) with the source's operations, association
+    and order as written: an evaluation independent of the oracle's code."""
+    eps = 1e-2
+    beta0 = (13.0 / 12.0) * ((v[0] - 2 * v[1] + v[2]) * (v[0] - 2 * v[1] + v[2])) \
+        + 0.25 * ((v[0] - 4 * v[1] + 3 * v[2]) * (v[0] - 4 * v[1] + 3 * v[2]))
+    beta1 = (13.0 / 12.0) * ((v[1] - 2 * v[2] + v[3]) * (v[1] - 2 * v[2] + v[3])) \
+        + 0.25 * ((v[1] - v[3]) * (v[1] - v[3]))
+    beta2 = (13.0 / 12.0) * ((v[2] - 2 * v[3] + v[4]) * (v[2] - 2 * v[3] + v[4])) \
+        + 0.25 * ((3 * v[2] - 4 * v[3] + v[4]) * (3 * v[2] - 4 * v[3] + v[4]))
+    alpha0 = 0.1 / ((eps + beta0) * (eps + beta0))
+    alpha1 = 0.6 / ((eps + beta1) * (eps + beta1))
+    alpha2 = 0.3 / ((eps + beta2) * (eps + beta2))
+    sum_alpha = alpha0 + alpha1 + alpha2
+    omega0 = alpha0 / sum_alpha
+    omega1 = alpha1 / sum_alpha
+    omega2 = alpha2 / sum_alpha
+    return (omega0 * (v[0] / 3.0 - 7.0 * v[1] / 6.0 + 11.0 * v[2] / 6.0)
+            + omega1 * (-v[1] / 6.0 + 5.0 * v[2] / 6.0 + v[3] / 3.0)
+            + omega2 * (v[2] / 3.0 + 5.0 * v[3] / 6.0 - v[4] / 6.0))
+
+
+def test_weno_bitwise_equal_to_the_source_order():
+    # a re-associated sum or a fused operation moves the result by an ulp or
+    # more on some of these inputs, which the 64-ulp rational bound lets pass
+    rng = np.random.default_rng(23)
+    n_diff = 0
+    for k in range(2000):
+        scale = 10.0 ** rng.integers(-4, 5)
+        v = (rng.normal(size=5) * scale).tolist()
+        if k % 4 == 0:
+            v = (v[0] + 1e-3 * scale * np.arange(5) ** 2).tolist()
+        if k % 7 == 0:
+            v[rng.integers(0, 5)] = 0.0
+        a, b = O.weno5(v), _weno5_source_order(v)
+        n_diff += math.copysign(1.0, a) != math.copysign(1.0, b) or a != b
+    assert n_diff == 0, n_diff
+
+
 # ---- Euler flux and HLL ------------------------------------------------------
 def test_euler_flux_known_answer():
     S = [2.0, 1.0, 0.5, 5.0]           # rho, rho u, rho v, E
