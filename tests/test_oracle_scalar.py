@@ -206,6 +206,77 @@ def test_hll_x_y_mirror():
         assert np.array_equal(fy, [fx[0], fx[2], fx[1], fx[3]])
 
 
+
+def _hll_source_order(g, L, R, xdir):
+    """EulerPhase::computePressurefromOther (EulerPhase.cpp:130-133),
+    EulerMod::comFlux (EulerMod.cpp:33-54) and RiemmanSolverHLL::Solver
+    (RiemmanSolverHLL.cpp:5-75) in Python floats, operations and order as
+    written; the "velocity" of the source is the momentum (std::min / max as
+    their ternaries)."""
+    def pressure(s):
+        rho, m0, m1, E = s
+        return (g - 1) * (E - 0.5 * (m0 * m0 + m1 * m1) / rho)
+
+    def flux(s, p):
+        rho, u, v, E = s
+        if xdir:
+            return [u, u * u / rho + p, u * v / rho, (E + p) * (u / rho)]
+        return [v, u * v / rho, v * v / rho + p, (E + p) * (v / rho)]
+
+    def sqrt(x):   # std::sqrt: NaN below zero
+        return math.sqrt(x) if x >= 0 else math.nan
+
+    pL, pR = pressure(L), pressure(R)
+    FL, FR = flux(L, pL), flux(R, pR)
+    rhoL, uL, EL = L[0], (L[1], L[2]), L[3]
+    rhoR, uR, ER = R[0], (R[1], R[2]), R[3]
+    aL = sqrt(G * pL / rhoL)
+    aR = sqrt(G * pR / rhoR)
+    i = 0 if xdir else 1
+    a_ave = 0.5 * (aL + aR)
+    rho_ave = 0.5 * (rhoL + rhoR)
+    p_star = 0.5 * (pL + pR) - 0.5 * (uR[i] / rhoR - uL[i] / rhoL) * rho_ave * a_ave
+    u_star = 0.5 * (uR[i] / rhoR + uL[i] / rhoL) - 0.5 * (pR - pL) / (rho_ave * a_ave)
+    rho_star_L = rhoL + (uL[i] / rhoL - u_star) * rho_ave / a_ave
+    rho_star_R = rhoR - (uR[i] / rhoR - u_star) * rho_ave / a_ave
+    a_star_L = sqrt(G * p_star / rho_star_L)
+    a_star_R = sqrt(G * p_star / rho_star_R)
+    x1, x2 = uL[i] - aL, u_star - a_star_L
+    SL = x2 if x2 < x1 else x1
+    y1, y2 = uL[i] + aL, u_star + a_star_R
+    SR = y2 if y1 < y2 else y1
+    if SL >= 0:
+        return FL
+    if SR <= 0:
+        return FR
+    inv = 1.0 / (SR - SL)
+    return [(SR * FL[0] - SL * FR[0] + SL * SR * (rhoR - rhoL)) * inv,
+            (SR * FL[1] - SL * FR[1] + SL * SR * (uR[0] - uL[0])) * inv,
+            (SR * FL[2] - SL * FR[2] + SL * SR * (uR[1] - uL[1])) * inv,
+            (SR * FL[3] - SL * FR[3] + SL * SR * (ER - EL)) * inv]
+
+
+def test_hll_bitwise_equal_to_the_source_order():
+    rng = np.random.default_rng(29)
+    branches = set()
+    for k in range(2000):
+        st = []
+        for _ in range(2):
+            rho = float(rng.uniform(0.1, 2.0))
+            u, v = (float(x) for x in rng.uniform(-3.0, 3.0, size=2))
+            p = float(10.0 ** rng.uniform(-1, 1))
+            st.append([rho, rho * u, rho * v, p / (G - 1) + 0.5 * rho * (u * u + v * v)])
+        for xdir in (True, False):
+            want = _hll_source_order(G, st[0], st[1], xdir)
+            if not all(math.isfinite(x) for x in want):
+                continue
+            got = O.hll(G, st[0], st[1], xdir)
+            assert [float(x) for x in got] == want, (st, xdir)
+            fl = _hll_source_order(G, st[0], st[0], xdir)
+            branches.add((want == fl, want == _hll_source_order(G, st[1], st[1], xdir)))
+    assert len(branches) >= 2   # both upwind and HLL-average branches were taken
+
+
 # ---- face walk ----------------------------------------------------------------
 def test_face_stencils_n8():
     # cells 0..7, ghost -1 and 8; the walk stops at boundary faces 0 and 8
