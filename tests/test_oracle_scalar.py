@@ -339,6 +339,49 @@ def test_rk_amplification_factors_of_linear_problem(z):
     assert _rk_linear(3, z) == pytest.approx(1 + 4 * z / 3 + z * z / 3 + z ** 3 / 6, rel=1e-14)
 
 
+
+def _rk_source_order(rk, u, rhs, dt):
+    """TVD2rdRK::advance (TVD2rdRK.cpp:3-31) and TVD3rdRK::advance
+    (TVD3rdRK.cpp:3-48) on one value over consecutive stage counters 1, 2, ...
+    (stage = counter % rk mapped as the source does), coefficient tables of
+    TVD2rdRK.h / TVD3rdRK.h, operations in the source's order."""
+    if rk == 2:
+        coef = [[0.0, 1.0, 1.0], [0.5, 0.5, 0.5]]
+        phase = None
+        for n, r in enumerate(rhs, start=1):
+            stage = 1 if n % 2 == 0 else 0
+            if stage == 0:
+                phase = u
+                u = coef[stage][1] * phase + coef[stage][2] * dt * r
+            else:
+                u = coef[stage][0] * phase + coef[stage][1] * u + coef[stage][2] * dt * r
+        return u
+    coef = [[0.0, 1.0, 1.0], [0.75, 0.25, 0.25], [1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0]]
+    phase0 = phase1 = None
+    for n, r in enumerate(rhs, start=1):
+        stage = 2 if n % 3 == 0 else 0 if n % 3 == 1 else 1
+        if stage == 0:
+            phase0 = u
+            u = coef[stage][1] * phase0 + coef[stage][2] * dt * r
+        elif stage == 1:
+            phase1 = u
+            u = coef[stage][0] * phase0 + coef[stage][1] * phase1 + coef[stage][2] * dt * r
+        else:
+            u = coef[stage][0] * phase0 + coef[stage][1] * phase1 + coef[stage][2] * dt * r
+    return u
+
+
+def test_rk_stages_bitwise_equal_to_the_source_order():
+    rng = np.random.default_rng(31)
+    for _ in range(2000):
+        u = float(rng.normal() * 10.0 ** rng.integers(-3, 4))
+        dt = float(10.0 ** rng.uniform(-6, -1))
+        rhs = (rng.normal(size=3) * 10.0 ** rng.integers(-2, 5)).tolist()
+        for rk in (2, 3):
+            for n in range(1, rk + 1):
+                assert O.rk_apply(rk, u, rhs[:n], dt) == _rk_source_order(rk, u, rhs[:n], dt), (rk, n, u, rhs, dt)
+
+
 # ---- checksum tree -----------------------------------------------------------
 def test_tree_sum_order():
     # the halving tree adds a0 + a2 first: (1e16 + -1e16) + 1 = 1, while a
