@@ -244,7 +244,8 @@ def run_batch(precision, count, poll_every=20, seed=BATCH_SEED, max_steps=20000)
             l1.append(float(np.abs(got[k][0] - s.get_state()[0]).mean()))
             s.close()
     stalled = sum(1 for st, _, _ in res.values() if st == cfs.SLOT_RUNNING)
-    cell_stages = sum(3 * res[k][1] * probs[k][0].nx * probs[k][0].ny for k in range(count))
+    # stages per step: the problems' RK order (TVD-RK2: 2)
+    cell_stages = sum(probs[k][0].rk * res[k][1] * probs[k][0].nx * probs[k][0].ny for k in range(count))
     # the same problems one after another, each alone
     t0 = time.perf_counter()
     seq_steps = 0
