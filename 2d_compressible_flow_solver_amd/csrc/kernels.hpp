@@ -1034,6 +1034,10 @@ template <typename R, bool VISC, bool LIMIT, typename Sh> struct Occupancy {
     static constexpr int lds = (int)sizeof(R) * (Sh::SR * Sh::SC * 4 + (Sh::NXF + Sh::NYF) * 4) + 8 * (Sh::NT / 64) + extra;
     static constexpr int by_lds = (160 * 1024 / lds) * (Sh::NT / 64) / 4;
     static constexpr int waves_per_simd = want < by_lds ? want : (by_lds > 0 ? by_lds : 1);
+    // the batched kernel also holds its problem's arguments, which at the
+    // single-solve figure spill: at most five waves (FP32) or four (FP64)
+    static constexpr int batch_cap = sizeof(R) == 8 ? 4 : 5;
+    static constexpr int batch_waves_per_simd = waves_per_simd < batch_cap ? waves_per_simd : batch_cap;
 };
 
 // The stage kernel: one workgroup per tile.  With 15 x 16 tiles: FP64 at
@@ -1074,7 +1078,7 @@ __global__ __launch_bounds__(Sh::NT, (Occupancy<R, VISC, LIMIT, Sh>::waves_per_s
 // from an atomic counter per range instead of the fixed stride was slower:
 // a range's workgroups reach the counter together.)
 template <typename R, bool LIMIT, typename Sh>
-__global__ __launch_bounds__(Sh::NT, (Occupancy<R, false, LIMIT, Sh>::waves_per_simd)) void batch_stage_kernel(
+__global__ __launch_bounds__(Sh::NT, (Occupancy<R, false, LIMIT, Sh>::batch_waves_per_simd)) void batch_stage_kernel(
     const StageArgs* __restrict__ descs, const int2* __restrict__ tiles, const int* __restrict__ ntiles, int parity,
     int sub) {
     __shared__ R S[Sh::SR * Sh::SC * 4];

@@ -185,7 +185,7 @@ BATCH_SEED = 2024
 BATCH_T_END = 0.2
 
 
-def run_batch(precision, count, poll_every=20, seed=BATCH_SEED, max_steps=20000):
+def run_batch(precision, count, poll_every=20, seed=BATCH_SEED, max_steps=20000, batch_only=False):
     """The batched workload: `count` seeded random Riemann problems of uneven
     size (N ~ U{64, 96, ..., 512}, transmissive boundaries, t_end = 0.2,
     CFL 0.5, TVD-RK2, positivity-preserving limiter) advanced together through the batch entry points until all
@@ -225,7 +225,7 @@ def run_batch(precision, count, poll_every=20, seed=BATCH_SEED, max_steps=20000)
     # accuracy of the batched FP32-bulk results: the first 16 problems again in
     # FP64 through the single-solve path, L1(rho) at the same end time
     l1 = []
-    if precision == cfs.PREC_FP32_BULK:
+    if precision == cfs.PREC_FP32_BULK and not batch_only:
         with cfs.Batch(rk=2, precision=precision, max_problems=16, limiter=True, tile_shape=shape,
                        arena_bytes=int(sum(cfs.Batch.footprint(c) for c, _ in probs[:16]) * 1.05) + (64 << 20)) as b2:
             slots = {b2.admit(c, u, cfl=CFL, t_end=BATCH_T_END): k for k, (c, u) in enumerate(probs[:16])}
@@ -249,7 +249,7 @@ def run_batch(precision, count, poll_every=20, seed=BATCH_SEED, max_steps=20000)
     # the same problems one after another, each alone
     t0 = time.perf_counter()
     seq_steps = 0
-    for k, (cfg, U) in enumerate(probs):
+    for k, (cfg, U) in enumerate([] if batch_only else probs):
         s = cfs.Solver(cfg)
         s.set_state(U)
         try:
@@ -271,9 +271,10 @@ def run_batch(precision, count, poll_every=20, seed=BATCH_SEED, max_steps=20000)
 
 def main_batch(args):
     prec = cfs.PREC_FP64 if args.batch_fp64 else cfs.PREC_FP32_BULK
-    r = run_batch(prec, args.batch_count)
-    print("batch: %d problems, %.2f s batched (%.1f Mcell-updates/s), %.2f s one after another (%.1f)"
-          % (r["problems"], r["wall"], r["mcups"], r["seq_wall"], r["seq_mcups"]), file=sys.stderr, flush=True)
+    r = run_batch(prec, args.batch_count, batch_only=args.batch_only)
+    print("batch: %d problems, %.2f s batched (%.1f Mcell-updates/s)" % (r["problems"], r["wall"], r["mcups"])
+          + ("" if args.batch_only else ", %.2f s one after another (%.1f)" % (r["seq_wall"], r["seq_mcups"])),
+          file=sys.stderr, flush=True)
     out = {
         "metric": "Mcell-updates/s per RK stage (batched solves)",
         "value": round(r["mcups"], 2),
@@ -289,9 +290,10 @@ def main_batch(args):
         "batched": {"wall_s": round(r["wall"], 3), "solves_per_s": round(r["solves_per_s"], 2),
                     "admission_s": round(r["admit_s"], 3), "failed": r["failed"], "stalled": r["stalled"],
                     "steps_range": [r["steps_min"], r["steps_max"]]},
-        "one_after_another": {"wall_s": round(r["seq_wall"], 3), "value": round(r["seq_mcups"], 2),
-                              "solves_per_s": round(r["seq_solves_per_s"], 2)},
-        "speedup_vs_one_after_another": round(r["seq_wall"] / r["wall"], 3),
+        "one_after_another": None if args.batch_only else {
+            "wall_s": round(r["seq_wall"], 3), "value": round(r["seq_mcups"], 2),
+            "solves_per_s": round(r["seq_solves_per_s"], 2)},
+        "speedup_vs_one_after_another": None if args.batch_only else round(r["seq_wall"] / r["wall"], 3),
     }
     if r["l1"]:
         out["l1_density_vs_fp64"] = {"problems": len(r["l1"]), "median": float(np.median(r["l1"])),
@@ -600,6 +602,8 @@ def main():
     ap.add_argument("--steps-per-poll", type=int, default=5)
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--batch-fp64", action="store_true", help="batched workload in FP64 instead of FP32-bulk")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="batched leg only (no one-after-another leg, no FP64 comparison; profiling)")
     ap.add_argument("--tile-shape", type=int, default=None, choices=range(cfs.NUM_TILE_SHAPES),
                     help="stage-kernel tile shape (default: picked by the autotuner)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="N",
