@@ -30,6 +30,12 @@ the same workload in the reference's MPI form (2D block decomposition, one
 rank per host thread up to 16, 3-cell ghost frames exchanged every stage), a
 bounded number of steps.
 
+The batch and shear-layer lines carry the same two objects: the batched
+kernel's roofline over the batched leg's wall time (a lower bound on the
+kernel's rate; its launches shrink as problems retire), the viscous kernel's
+over its launch duration by HIP events; their CPU baselines run the
+restatement on the same problems for 20 steps each within about 12 s.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
        python bench.py --workload batch | serve    (batched / serving measurements)
        python bench.py --workload kh               (viscous shear layer 4096^2 to t = 1, FP64 and FP32-bulk)
@@ -57,6 +63,7 @@ CFL = 0.5
 FREESTREAM = (1.4, 3.0, 0.0, 1.0)           # Mach 3 at c = 1
 REFERENCE_INFLOW = (1.0, 1.0, 0.0, 1.0)     # Cylinder.cpp:5-16
 BYTES_PER_CELL_STAGE = 97.0
+BYTES_PER_CELL_STAGE_RK3 = 96.0   # RK3 mean without the cell-type byte (no immersed body)
 
 
 def kernel_name(prec, limiter, shape):
@@ -85,6 +92,55 @@ def profile_figure(key):
         if line.startswith(key + ":"):
             return float(line.split(":", 1)[1])
     return None
+
+
+def profile_row(name, label, nth=0, col=2):
+    """Cell `col` (2: the second) of the nth table row whose first cell starts
+    with `label` in the committed summary profiles/<name>, as a float (its
+    first number); None when absent."""
+    p = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(p):
+        return None
+    rows = [l for l in open(p) if l.startswith("| " + label)]
+    if len(rows) <= nth:
+        return None
+    cell = rows[nth].split("|")[col].strip()
+    try:
+        return float(cell.split()[0].replace(",", ""))
+    except ValueError:
+        return None
+
+
+def cpu_window(problems, steps_each, budget_s=12.0):
+    """The strict-IEEE CPU restatement (oracle/) in the reference's MPI form on
+    a bounded sample: each (cfg, U, ib) of `problems` in turn for `steps_each`
+    CFL steps (FP64, the problem's own RK order), until budget_s is spent
+    (at least one step).  Returns (cell-stages, wall seconds, steps, threads,
+    decompositions used)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as O
+    nthreads = max(1, min(16, os.cpu_count() or 1))
+    cell_stages, steps, dims_seen, wall = 0, 0, set(), 0.0
+    for cfg, U, ib in problems:
+        c64 = cfs.Config.from_buffer_copy(cfg)
+        c64.precision = cfs.PREC_FP64
+        o = O.Oracle(c64, nthreads=nthreads)
+        dims_seen.add(tuple(o.set_ranks(min(nthreads, c64.nx * c64.ny // 64))))
+        if ib is not None:
+            o.set_ib_table(ib)
+        o.set_state(U)
+        t0 = time.perf_counter()
+        for _ in range(steps_each):
+            o.step(CFL / o.cfl_smax())
+            steps += 1
+            cell_stages += c64.nx * c64.ny * c64.rk
+            if wall + time.perf_counter() - t0 > budget_s:
+                break
+        wall += time.perf_counter() - t0
+        o.close()
+        if wall > budget_s:
+            break
+    return cell_stages, wall, steps, nthreads, sorted(dims_seen)
 
 
 def setup(precision, reference_scheme=False):
@@ -183,6 +239,9 @@ def run_cpu_baseline(budget_s=12.0):
 
 BATCH_SEED = 2024
 BATCH_T_END = 0.2
+BATCH_BYTES_PER_CELL_STAGE = 80    # TVD-RK2: stage 0 reads 32 B, stage 1 reads 64 B, each writes 32 B
+BATCH_PROFILE = "batch_stage_kernel_riemann_batch_256.md"
+KH_PROFILE = "stage_kernel_viscous_shear_layer_4096x4096.md"
 
 
 def run_batch(precision, count, poll_every=20, seed=BATCH_SEED, max_steps=20000, batch_only=False):
@@ -298,6 +357,38 @@ def main_batch(args):
     if r["l1"]:
         out["l1_density_vs_fp64"] = {"problems": len(r["l1"]), "median": float(np.median(r["l1"])),
                                      "max": float(np.max(r["l1"]))}
+    # roofline of the batched stage kernel over the batched leg's wall time
+    # (admission, polls and retirements included: a lower bound on the
+    # kernel's own rate; its launches shrink as problems retire)
+    alg = BATCH_BYTES_PER_CELL_STAGE * r["cell_stages"]
+    achieved = alg / r["wall"] / 1e9
+    out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                       "timed_over": "batched leg wall time",
+                       "kernel": "batch_stage_kernel<%s, true, TileShape<%d, %d, %d>>" % (
+                           ("double" if prec == cfs.PREC_FP64 else "float",) + tuple(cfs.tile_shape_info(r["shape"]))),
+                       "algorithmic_bytes_per_cell_stage": BATCH_BYTES_PER_CELL_STAGE}
+    if prec == cfs.PREC_FP32_BULK and r["problems"] == 256:
+        ratio = profile_row(BATCH_PROFILE, "traffic / algorithmic")
+        lane_ops = profile_row(BATCH_PROFILE, "VALU lane-ops per cell-stage")
+        if ratio:
+            out["roofline"]["traffic_over_algorithmic"] = ratio
+        if lane_ops:
+            # FP32 without packed math issues at the FP64 lane rate
+            v = lane_ops * r["cell_stages"] / r["wall"] / 1e12
+            out["roofline"]["valu"] = {"achieved": round(v, 2), "peak": FP64_VALU_PEAK_T, "unit": "T lane-ops/s",
+                                       "frac": round(v / FP64_VALU_PEAK_T, 4), "lane_ops_per_cell_stage": lane_ops}
+    out["cpu_baseline"] = None
+    if not args.no_cpu and not args.batch_only:
+        probs = cfs.riemann_batch_problems(BATCH_SEED, count=args.batch_count, precision=cfs.PREC_FP64, rk=2)
+        cs, wall, steps, nthreads, dims = cpu_window([(c, u, None) for c, u in probs], 20)
+        out["cpu_baseline"] = {
+            "value": round(cs / wall / 1e6, 3), "unit": "Mcell-updates/s", "cores": nthreads, "kind": "port",
+            "sample": "%d RK2 steps of the batch's problems in order, 20 each (%.1f s), FP64, strict-IEEE CPU "
+                      "restatement of the reference in its MPI form, one rank per thread, blocks %s"
+                      % (steps, wall, ", ".join("%dx%d" % d for d in dims))}
+        print("cpu baseline: %.2f Mcell-updates/s on %d threads" % (out["cpu_baseline"]["value"], nthreads),
+              file=sys.stderr, flush=True)
     print(json.dumps(out))
 
 
@@ -463,6 +554,30 @@ def main_kh(args):
         "l1_density_fp32_bulk_vs_fp64": l1,
         "linf_density_fp32_bulk_vs_fp64": float(np.abs(r32["rho"] - r64["rho"]).max()),
     }
+    # roofline of the FP64 viscous stage kernel (launch duration by HIP events
+    # at t_end; RK3 mean 96 B per cell-stage)
+    bytes_per_launch = n * n * BYTES_PER_CELL_STAGE_RK3
+    if r64["kernel_ms"] > 0:
+        achieved = bytes_per_launch / (r64["kernel_ms"] * 1e-3) / 1e9
+        out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(achieved / HBM_PEAK_GBS, 4),
+                           "traffic": profile_row(KH_PROFILE, "traffic (sum)") if n == 4096 else None,
+                           "kernel": "stage_kernel<double, true, false, ...>",
+                           "kernel_ms": round(r64["kernel_ms"], 5), "algorithmic_bytes_per_launch": bytes_per_launch}
+        lane_ops = profile_row(KH_PROFILE, "SQ_INSTS_VALU", col=3) if n == 4096 else None
+        if lane_ops:
+            v = lane_ops * n * n / (r64["kernel_ms"] * 1e-3) / 1e12
+            out["roofline"]["valu"] = {"achieved": round(v, 2), "peak": FP64_VALU_PEAK_T, "unit": "T lane-ops/s",
+                                       "frac": round(v / FP64_VALU_PEAK_T, 4), "lane_ops_per_cell_stage": lane_ops}
+    out["cpu_baseline"] = None
+    if not args.no_cpu:
+        cfg = cfs.config_kelvin_helmholtz(n, precision=cfs.PREC_FP64)
+        cs, wall, steps, nthreads, dims = cpu_window([(cfg, cfs.kelvin_helmholtz(cfg), None)], 20)
+        out["cpu_baseline"] = {
+            "value": round(cs / wall / 1e6, 3), "unit": "Mcell-updates/s", "cores": nthreads, "kind": "port",
+            "sample": "%d RK3 steps of the same %dx%d shear layer from its initial state (%.1f s), strict-IEEE CPU "
+                      "restatement of the reference in its MPI form, %dx%d blocks, one rank per thread"
+                      % (steps, n, n, wall, dims[0][0], dims[0][1])}
     if args.kh_control:
         # the flow's own sensitivity: FP64 from a density perturbed by 1e-14
         # (relative), compared with the FP64 run at the same times
