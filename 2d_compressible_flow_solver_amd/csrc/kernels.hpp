@@ -742,77 +742,91 @@ __device__ __forceinline__ void viscous_flux(const ViscLDS<R, Sh>& V, const int 
         out[q] = ((R(-1.0) / R(24.0)) * acc[0][q] + (R(26.0) / R(24.0)) * acc[1][q]) + (R(-1.0) / R(24.0)) * acc[2][q];
 }
 
+// One face of a tile, flux to LDS.  s indexes the tile's faces: x-faces
+// [0, NXF) (face i0 + s / TJ at column s % TJ), then y-faces [NXF, NXF + NYF).
+// DIR = 1 or 2 fixes the face's direction at compile time (its LDS strides and
+// flux components become constants); DIR = 0 decides it from s at run time
+// (one body for both directions: fewer registers, more index arithmetic).
+template <typename R, bool VISC, bool LIMIT, bool FAST, typename Sh, int DIR>
+__device__ __forceinline__ void tile_face(const StageArgs& a, int s, int i0, int j0, R g, R lam_x, R lam_y,
+                                          const R* S, R* F, const ViscLDS<R, Sh>* V, const LimitLDS<R, Sh>* Lm) {
+    constexpr int TJ = Sh::TJ, SC = Sh::SC, NXF = Sh::NXF;
+    constexpr int NS = Sh::SR * Sh::SC, NF = NXF + Sh::NYF;   // LDS plane lengths (state, fluxes)
+    const bool xf = DIR == 0 ? s < NXF : DIR == 1;
+    const int nrm = xf ? 1 : 2, tng = xf ? 2 : 1;
+    int cell[6], gcell[4];
+    if (xf) {
+        const bool px = a.bc[0] == BC_PERIODIC;
+        const int fr = s / TJ, jl = s - fr * TJ;
+        const int f = i0 + fr;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            int o = k - 3;
+            // boundary faces: the walk stops at the boundary face, so the
+            // stencil is [G,G,G,c0,c0,c0] at f = 0 and [c,c,c,G,G,G] at f = n
+            if (!px && f == 0 && o > 0) o = 0;
+            if (!px && f == a.nx && o < -1) o = -1;
+            cell[k] = (fr + o + HALO) * SC + jl + HALO;
+            if (VISC && k >= 1 && k <= 4) gcell[k - 1] = (fr + o + HALO - 1) * ViscLDS<R, Sh>::GYW + jl;
+        }
+    } else {
+        const bool py = a.bc[2] == BC_PERIODIC;
+        const int t = s - NXF;
+        const int il = t / (TJ + 1), gl = t - il * (TJ + 1);
+        const int gf = j0 + gl;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            int o = k - 3;
+            if (!py && gf == 0 && o > 0) o = 0;
+            if (!py && gf == a.ny && o < -1) o = -1;
+            cell[k] = (il + HALO) * SC + gl + o + HALO;
+            if (VISC && k >= 1 && k <= 4) gcell[k - 1] = il * ViscLDS<R, Sh>::GXW + gl + o + HALO - 1;
+        }
+    }
+    R fl[4];
+    face_flux<R, FAST>(S, NS, cell, nrm, tng, g, a.positivity, a.hll_bounds, fl);
+    if constexpr (LIMIT) {
+        if (a.positivity == 2)
+            limit_face<R, Sh>(S, NS, *Lm, cell[2], cell[3], nrm, tng, xf ? SC : 1, xf ? 1 : SC,
+                              xf ? lam_x : lam_y, xf ? lam_y : lam_x, g, fl);
+    }
+    if constexpr (VISC) {   // net flux = inviscid - viscous (mass: minus zero)
+        R fv[3];
+        viscous_flux<R, Sh>(*V, cell, gcell, xf, xf ? (R)a.dx : (R)a.dy, xf ? a.dx_pow2 != 0 : a.dy_pow2 != 0,
+                            xf ? (R)a.inv_dx : (R)a.inv_dy, (R)a.mu, (R)a.kappa, fv);
+        fl[1] = fl[1] - fv[0];
+        fl[2] = fl[2] - fv[1];
+        fl[3] = fl[3] - fv[2];
+    }
+    F[s] = fl[0];
+    F[nrm * NF + s] = fl[1];
+    F[tng * NF + s] = fl[2];
+    F[3 * NF + s] = fl[3];
+}
+
 // Phase 1 of a tile: its 511 faces, fluxes to LDS.  FAST selects the WENO5
-// division forms (WOps); the caller picks it per tile.
-template <typename R, bool VISC, bool LIMIT, bool FAST, typename Sh>
+// division forms (WOps); the caller picks it per tile.  SPLIT: x-faces, then
+// y-faces, each direction its own code; else one loop over both.
+template <typename R, bool VISC, bool LIMIT, bool FAST, typename Sh, bool SPLIT>
 __device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, double dt, const R* S, R* F,
                                            const ViscLDS<R, Sh>* V, const LimitLDS<R, Sh>* Lm) {
-    constexpr int TJ = Sh::TJ, SC = Sh::SC, NXF = Sh::NXF, NYF = Sh::NYF, BLOCK = Sh::NT;
-    constexpr int NS = Sh::SR * Sh::SC, NF = NXF + NYF;   // LDS plane lengths (state, fluxes)
+    constexpr int NXF = Sh::NXF, NF = Sh::NXF + Sh::NYF;
     const int tid = threadIdx.x;
-    const bool px = a.bc[0] == BC_PERIODIC, py = a.bc[2] == BC_PERIODIC;
     const R g = (R)a.gamma;
     // Lax-Friedrichs shares of the limiter: dt/dx and dt/dy
     const R lam_x = (R)(dt / a.dx), lam_y = (R)(dt / a.dy);
-    for (int s = tid; s < NXF + NYF; s += BLOCK) {
-        int cell[6], gcell[4], nrm, tng;
-        if (s < NXF) {
-            const int fr = s / TJ, jl = s - fr * TJ;
-            const int f = i0 + fr;
-#pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                int o = k - 3;
-                // boundary faces: the walk stops at the boundary face, so the
-                // stencil is [G,G,G,c0,c0,c0] at f = 0 and [c,c,c,G,G,G] at f = n
-                if (!px && f == 0 && o > 0) o = 0;
-                if (!px && f == a.nx && o < -1) o = -1;
-                cell[k] = (fr + o + HALO) * SC + jl + HALO;
-                if (VISC && k >= 1 && k <= 4) gcell[k - 1] = (fr + o + HALO - 1) * ViscLDS<R, Sh>::GYW + jl;
-            }
-            nrm = 1; tng = 2;
-        } else {
-            const int t = s - NXF;
-            const int il = t / (TJ + 1), gl = t - il * (TJ + 1);
-            const int gf = j0 + gl;
-#pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                int o = k - 3;
-                if (!py && gf == 0 && o > 0) o = 0;
-                if (!py && gf == a.ny && o < -1) o = -1;
-                cell[k] = (il + HALO) * SC + gl + o + HALO;
-                if (VISC && k >= 1 && k <= 4) gcell[k - 1] = il * ViscLDS<R, Sh>::GXW + gl + o + HALO - 1;
-            }
-            nrm = 2; tng = 1;
-        }
-        R fl[4];
-        face_flux<R, FAST>(S, NS, cell, nrm, tng, g, a.positivity, a.hll_bounds, fl);
-        if constexpr (LIMIT) {
-            if (a.positivity == 2) {
-                const bool xf = s < NXF;
-                limit_face<R, Sh>(S, NS, *Lm, cell[2], cell[3], nrm, tng, xf ? SC : 1, xf ? 1 : SC,
-                              xf ? lam_x : lam_y, xf ? lam_y : lam_x, g, fl);
-            }
-        }
-        if constexpr (VISC) {   // net flux = inviscid - viscous (mass: minus zero)
-            int cs[6];
-#pragma unroll
-            for (int k = 0; k < 6; ++k) cs[k] = cell[k];
-            const bool xface = s < NXF;
-            R fv[3];
-            viscous_flux<R, Sh>(*V, cs, gcell, xface, xface ? (R)a.dx : (R)a.dy, xface ? a.dx_pow2 != 0 : a.dy_pow2 != 0,
-                                xface ? (R)a.inv_dx : (R)a.inv_dy, (R)a.mu, (R)a.kappa, fv);
-            fl[1] = fl[1] - fv[0];
-            fl[2] = fl[2] - fv[1];
-            fl[3] = fl[3] - fv[2];
-        }
-        F[s] = fl[0];
-        F[nrm * NF + s] = fl[1];
-        F[tng * NF + s] = fl[2];
-        F[3 * NF + s] = fl[3];
+    if constexpr (SPLIT) {
+        for (int s = tid; s < NXF; s += Sh::NT)
+            tile_face<R, VISC, LIMIT, FAST, Sh, 1>(a, s, i0, j0, g, lam_x, lam_y, S, F, V, Lm);
+        for (int s = NXF + tid; s < NF; s += Sh::NT)
+            tile_face<R, VISC, LIMIT, FAST, Sh, 2>(a, s, i0, j0, g, lam_x, lam_y, S, F, V, Lm);
+    } else {
+        for (int s = tid; s < NF; s += Sh::NT)
+            tile_face<R, VISC, LIMIT, FAST, Sh, 0>(a, s, i0, j0, g, lam_x, lam_y, S, F, V, Lm);
     }
 }
 
-template <typename R, bool VISC, bool LIMIT, typename Sh>
+template <typename R, bool VISC, bool LIMIT, typename Sh, bool SPLIT = true>
 __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R* F, double* wmax,
                                            ViscLDS<R, Sh>* V, LimitLDS<R, Sh>* Lm) {
     constexpr int TI = Sh::TI, TJ = Sh::TJ, SR = Sh::SR, SC = Sh::SC, NXF = Sh::NXF, BLOCK = Sh::NT;
@@ -968,9 +982,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
 
     // ---- faces: the exact-division form on tiles outside the fast range ----
     if (sizeof(R) == 8 && tile_unsafe)
-        tile_faces<R, VISC, LIMIT, false, Sh>(a, i0, j0, dt, S, F, V, Lm);
+        tile_faces<R, VISC, LIMIT, false, Sh, SPLIT>(a, i0, j0, dt, S, F, V, Lm);
     else
-        tile_faces<R, VISC, LIMIT, true, Sh>(a, i0, j0, dt, S, F, V, Lm);
+        tile_faces<R, VISC, LIMIT, true, Sh, SPLIT>(a, i0, j0, dt, S, F, V, Lm);
     __syncthreads();
 
     // ---- cell update: flux divergence (Cell::comRHS, Cell.cpp:110-125) and the
@@ -1038,6 +1052,11 @@ template <typename R, bool VISC, bool LIMIT, typename Sh> struct Occupancy {
     // single-solve figure spill: at most five waves (FP32) or four (FP64)
     static constexpr int batch_cap = sizeof(R) == 8 ? 4 : 5;
     static constexpr int batch_waves_per_simd = waves_per_simd < batch_cap ? waves_per_simd : batch_cap;
+    // faces of the batched kernel, one body per direction in FP64 (8,138 ->
+    // 8,363 Mcell-updates/s on 256 Riemann problems); FP32 keeps one loop over
+    // both (0.3% faster: split, it spills at five waves), and so does the
+    // 1024-thread shape (split, it spills in FP64)
+    static constexpr bool batch_split = sizeof(R) == 8 && Sh::NT <= 512;
 };
 
 // The stage kernel: one workgroup per tile.  With 15 x 16 tiles: FP64 at
@@ -1092,11 +1111,13 @@ __global__ __launch_bounds__(Sh::NT, (Occupancy<R, false, LIMIT, Sh>::batch_wave
     const int lo = (int)((n * x) >> 3), hi = (int)((n * (x + 1)) >> 3);
     for (int t = lo + l; t < hi; t += per) {
         const int2 pt = tiles[t];
+        constexpr bool split = Occupancy<R, false, LIMIT, Sh>::batch_split;
         if constexpr (LIMIT)
-            stage_tile<R, false, true, Sh>(descs[(pt.x * 2 + parity) * 3 + sub], pt.y, S, F, wmax, nullptr, &Lm);
+            stage_tile<R, false, true, Sh, split>(descs[(pt.x * 2 + parity) * 3 + sub], pt.y, S, F, wmax, nullptr,
+                                                  &Lm);
         else
-            stage_tile<R, false, false, Sh>(descs[(pt.x * 2 + parity) * 3 + sub], pt.y, S, F, wmax, nullptr,
-                                            nullptr);
+            stage_tile<R, false, false, Sh, split>(descs[(pt.x * 2 + parity) * 3 + sub], pt.y, S, F, wmax, nullptr,
+                                                   nullptr);
         __syncthreads();   // the next tile reuses S, F and wmax
     }
 }
