@@ -624,24 +624,31 @@ int32_t cfs_autotune(cfs_solver* s, int32_t reps, int32_t* best, double* ms) {
     if (!rc) rc = set_dt_device(s, 0.0);
     // sub-stage 0 of the current state, fixed dt = 0, into plane-set 1: the
     // stage kernel's whole work (tile load with the boundary and IB fills,
-    // faces, update, stores) without changing the solve
-    for (int k = 0; k < nshape && !rc; ++k) {
-        apply_shape(s, k);
-        StageArgs a;
-        fill_stage_args(a, s->dev, 0, s->step_parity, 0, StageTime{});
-        a.exact_count = cnt;
-        a.exact_other = cnt + 1;
-        const dim3 grid = stage_grid(s);
-        for (int w = 0; w < 2; ++w) dispatch_stage(s->cfg, k, grid, s->stream, a);
-        if (hipEventRecord(e0, s->stream) != hipSuccess) { rc = CFS_E_HIP; break; }
-        for (int n = 0; n < reps; ++n) dispatch_stage(s->cfg, k, grid, s->stream, a);
-        float msk = 0.f;
-        if (hipGetLastError() != hipSuccess || hipEventRecord(e1, s->stream) != hipSuccess ||
-            hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&msk, e0, e1) != hipSuccess) {
-            rc = CFS_E_HIP;
-            break;
+    // faces, update, stores) without changing the solve.  Two passes over the
+    // shapes, each shape's faster one kept: the first shape timed on an idle
+    // GPU reads slow while the clocks ramp up (5-7% on a 0.25 ms stage).
+    for (int pass = 0; pass < 2 && !rc; ++pass) {
+        for (int k = 0; k < nshape && !rc; ++k) {
+            apply_shape(s, k);
+            StageArgs a;
+            fill_stage_args(a, s->dev, 0, s->step_parity, 0, StageTime{});
+            a.exact_count = cnt;
+            a.exact_other = cnt + 1;
+            const dim3 grid = stage_grid(s);
+            for (int w = 0; w < 2; ++w) dispatch_stage(s->cfg, k, grid, s->stream, a);
+            if (hipEventRecord(e0, s->stream) != hipSuccess) { rc = CFS_E_HIP; break; }
+            for (int n = 0; n < reps; ++n) dispatch_stage(s->cfg, k, grid, s->stream, a);
+            float msk = 0.f;
+            if (hipGetLastError() != hipSuccess || hipEventRecord(e1, s->stream) != hipSuccess ||
+                hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&msk, e0, e1) != hipSuccess) {
+                rc = CFS_E_HIP;
+                break;
+            }
+            const double tk = msk / reps;
+            if (pass == 0 || tk < t[k]) t[k] = tk;
         }
-        t[k] = msk / reps;
+    }
+    for (int k = 0; k < nshape && !rc; ++k) {
         if (ms) ms[k] = t[k];
         if (k == 0 || t[k] < t[k_best]) k_best = k;
     }

@@ -249,10 +249,11 @@ int32_t cfs_set_tile_shape(cfs_solver* s, int32_t shape);
 int32_t cfs_get_tile_shape(const cfs_solver* s, int32_t* shape);
 /* Autotuner: times `reps` launches of the stage kernel of every tile shape on
  * the handle's current state and configuration (sub-stage 0 with dt = 0
- * written to a spare plane-set: the state is not changed), selects the
- * fastest and reports it in *best (may be NULL) and the mean milliseconds per
- * launch of each shape in ms[CFS_NUM_TILE_SHAPES] (may be NULL; -1 for shapes
- * the configuration has not).  At a step boundary only (CFS_E_STATE). */
+ * written to a spare plane-set: the state is not changed), in two passes
+ * over the shapes, selects the fastest and reports it in *best (may be NULL)
+ * and the mean milliseconds per launch of each shape's faster pass in
+ * ms[CFS_NUM_TILE_SHAPES] (may be NULL; -1 for shapes the configuration has
+ * not).  At a step boundary only (CFS_E_STATE). */
 int32_t cfs_autotune(cfs_solver* s, int32_t reps, int32_t* best, double* ms);
 
 /* ---- batched solves ------------------------------------------------------- */
