@@ -1039,8 +1039,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
 // Waves per SIMD the stage kernels are compiled for: the second argument of
 // __launch_bounds__ is the minimum number of waves per execution unit (SIMD)
 // on this target, which sets the VGPR budget (512 / waves).  The wanted
-// figure -- FP64 5, FP32 6, with the limiter FP64 4 (105 VGPRs), viscous FP64
-// 3 (147 VGPRs) and FP32 6 -- is capped by the waves per SIMD that the
+// figure -- FP64 5, FP32 6, with the limiter FP64 4 (95 VGPRs), viscous FP64
+// 3 (118 VGPRs) and FP32 6 -- is capped by the waves per SIMD that the
 // workgroup's LDS (160 KB per CU) allows.
 template <typename R, bool VISC, bool LIMIT, typename Sh> struct Occupancy {
     static constexpr int want = VISC ? (sizeof(R) == 8 ? 3 : 6) : LIMIT ? (sizeof(R) == 8 ? 4 : 6) : (sizeof(R) == 8 ? 5 : 6);
