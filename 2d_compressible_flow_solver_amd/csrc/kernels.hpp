@@ -57,6 +57,10 @@ template <int TI_, int TJ_, int NT_> struct TileShape {
     static constexpr int SR = TI + 2 * HALO, SC = TJ + 2 * HALO;
     static constexpr int NXF = (TI + 1) * TJ, NYF = TI * (TJ + 1);
     static_assert(NXF + NYF <= 2 * NT && TI * TJ <= NT && NT % 64 == 0, "two face rounds, one cell per thread");
+    static_assert(NXF >= NYF, "the y-faces fit in the x-faces' flux planes");
+    // flux plane length in LDS: every face, or (HALF) the x-faces only, which
+    // the y-faces overwrite once each cell holds its x part of the divergence
+    template <bool HALF> static constexpr int flux_plane() { return HALF ? NXF : NXF + NYF; }
 };
 using Shape0 = TileShape<15, 16, 256>;    // 240 cells, 511 faces (256 + 255)
 using Shape1 = TileShape<15, 32, 512>;    // 480 cells, 1007 faces (512 + 495)
@@ -747,11 +751,11 @@ __device__ __forceinline__ void viscous_flux(const ViscLDS<R, Sh>& V, const int 
 // DIR = 1 or 2 fixes the face's direction at compile time (its LDS strides and
 // flux components become constants); DIR = 0 decides it from s at run time
 // (one body for both directions: fewer registers, more index arithmetic).
-template <typename R, bool VISC, bool LIMIT, bool FAST, typename Sh, int DIR>
+template <typename R, bool VISC, bool LIMIT, bool FAST, typename Sh, int DIR, bool HALF>
 __device__ __forceinline__ void tile_face(const StageArgs& a, int s, int i0, int j0, R g, R lam_x, R lam_y,
                                           const R* S, R* F, const ViscLDS<R, Sh>* V, const LimitLDS<R, Sh>* Lm) {
     constexpr int TJ = Sh::TJ, SC = Sh::SC, NXF = Sh::NXF;
-    constexpr int NS = Sh::SR * Sh::SC, NF = NXF + Sh::NYF;   // LDS plane lengths (state, fluxes)
+    constexpr int NS = Sh::SR * Sh::SC, NF = Sh::template flux_plane<HALF>();   // LDS plane lengths
     const bool xf = DIR == 0 ? s < NXF : DIR == 1;
     const int nrm = xf ? 1 : 2, tng = xf ? 2 : 1;
     int cell[6], gcell[4];
@@ -798,18 +802,36 @@ __device__ __forceinline__ void tile_face(const StageArgs& a, int s, int i0, int
         fl[2] = fl[2] - fv[1];
         fl[3] = fl[3] - fv[2];
     }
-    F[s] = fl[0];
-    F[nrm * NF + s] = fl[1];
-    F[tng * NF + s] = fl[2];
-    F[3 * NF + s] = fl[3];
+    // the face's slot: x-faces [0, NXF), y-faces after them or, with HALF
+    // (the y-faces written over the x-faces once those are consumed), from 0
+    const int e = (HALF && !xf) ? s - NXF : s;
+    F[e] = fl[0];
+    F[nrm * NF + e] = fl[1];
+    F[tng * NF + e] = fl[2];
+    F[3 * NF + e] = fl[3];
+}
+
+// x-face part of a cell's flux divergence, -(F(i+1/2) - F(i-1/2)) / dx, the
+// first operand of Cell::comRHS's difference (Cell.cpp:110-125)
+template <typename R, typename Sh, bool HALF>
+__device__ __forceinline__ R rhs_x(const StageArgs& a, const R* F, int q, int il, int jl) {
+    constexpr int TJ = Sh::TJ, NF = Sh::template flux_plane<HALF>();
+    const R fl = F[q * NF + il * TJ + jl];
+    const R fr = F[q * NF + (il + 1) * TJ + jl];
+    // x/d == x*(1/d) bit for bit when d is a power of two
+    return a.dx_pow2 ? (-(fr - fl)) * (R)a.inv_dx : K<R>::div(-(fr - fl), (R)a.dx);
 }
 
 // Phase 1 of a tile: its 511 faces, fluxes to LDS.  FAST selects the WENO5
 // division forms (WOps); the caller picks it per tile.  SPLIT: x-faces, then
-// y-faces, each direction its own code; else one loop over both.
-template <typename R, bool VISC, bool LIMIT, bool FAST, typename Sh, bool SPLIT>
+// y-faces, each direction its own code; else one loop over both.  HALF (with
+// SPLIT): one flux plane-set of NXF faces, which the y-faces overwrite after
+// each cell has taken the x part of its divergence into tx (the cells of
+// threads tid < TI * TJ).
+template <typename R, bool VISC, bool LIMIT, bool FAST, typename Sh, bool SPLIT, bool HALF>
 __device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, double dt, const R* S, R* F,
-                                           const ViscLDS<R, Sh>* V, const LimitLDS<R, Sh>* Lm) {
+                                           const ViscLDS<R, Sh>* V, const LimitLDS<R, Sh>* Lm, R tx[4]) {
+    static_assert(!HALF || SPLIT, "the halved flux planes need the x-faces first");
     constexpr int NXF = Sh::NXF, NF = Sh::NXF + Sh::NYF;
     const int tid = threadIdx.x;
     const R g = (R)a.gamma;
@@ -817,20 +839,31 @@ __device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, d
     const R lam_x = (R)(dt / a.dx), lam_y = (R)(dt / a.dy);
     if constexpr (SPLIT) {
         for (int s = tid; s < NXF; s += Sh::NT)
-            tile_face<R, VISC, LIMIT, FAST, Sh, 1>(a, s, i0, j0, g, lam_x, lam_y, S, F, V, Lm);
+            tile_face<R, VISC, LIMIT, FAST, Sh, 1, HALF>(a, s, i0, j0, g, lam_x, lam_y, S, F, V, Lm);
+        if constexpr (HALF) {
+            __syncthreads();
+            if (tid < Sh::TI * Sh::TJ) {
+                const int il = tid / Sh::TJ, jl = tid - il * Sh::TJ;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) tx[q] = rhs_x<R, Sh, HALF>(a, F, q, il, jl);
+            }
+            __syncthreads();
+        }
         for (int s = NXF + tid; s < NF; s += Sh::NT)
-            tile_face<R, VISC, LIMIT, FAST, Sh, 2>(a, s, i0, j0, g, lam_x, lam_y, S, F, V, Lm);
+            tile_face<R, VISC, LIMIT, FAST, Sh, 2, HALF>(a, s, i0, j0, g, lam_x, lam_y, S, F, V, Lm);
     } else {
         for (int s = tid; s < NF; s += Sh::NT)
-            tile_face<R, VISC, LIMIT, FAST, Sh, 0>(a, s, i0, j0, g, lam_x, lam_y, S, F, V, Lm);
+            tile_face<R, VISC, LIMIT, FAST, Sh, 0, HALF>(a, s, i0, j0, g, lam_x, lam_y, S, F, V, Lm);
     }
 }
 
-template <typename R, bool VISC, bool LIMIT, typename Sh, bool SPLIT = true>
+template <typename R, bool VISC, bool LIMIT, typename Sh, bool SPLIT = true, bool HALF = false>
 __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R* F, double* wmax,
                                            ViscLDS<R, Sh>* V, LimitLDS<R, Sh>* Lm) {
     constexpr int TI = Sh::TI, TJ = Sh::TJ, SR = Sh::SR, SC = Sh::SC, NXF = Sh::NXF, BLOCK = Sh::NT;
-    constexpr int NS = SR * SC, NF = NXF + Sh::NYF;   // LDS plane lengths (state, fluxes)
+    constexpr int NS = SR * SC, NF = Sh::template flux_plane<HALF>();   // LDS plane lengths (state, fluxes)
+    // y-faces' first slot in the flux planes
+    constexpr int YF = HALF ? 0 : NXF;
     const int tid = threadIdx.x;
     const int ti = tile / a.tiles_j;
     const int tj = tile - ti * a.tiles_j;
@@ -981,10 +1014,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
     }
 
     // ---- faces: the exact-division form on tiles outside the fast range ----
+    R tx[4];   // with HALF: the x part of the cell's flux divergence
     if (sizeof(R) == 8 && tile_unsafe)
-        tile_faces<R, VISC, LIMIT, false, Sh, SPLIT>(a, i0, j0, dt, S, F, V, Lm);
+        tile_faces<R, VISC, LIMIT, false, Sh, SPLIT, HALF>(a, i0, j0, dt, S, F, V, Lm, tx);
     else
-        tile_faces<R, VISC, LIMIT, true, Sh, SPLIT>(a, i0, j0, dt, S, F, V, Lm);
+        tile_faces<R, VISC, LIMIT, true, Sh, SPLIT, HALF>(a, i0, j0, dt, S, F, V, Lm, tx);
     __syncthreads();
 
     // ---- cell update: flux divergence (Cell::comRHS, Cell.cpp:110-125) and the
@@ -996,8 +1030,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
         if (i < a.nx && j < a.ny) {
             const long long c = (long long)i * a.ny + j;
             const int ty = (a.type == nullptr) ? 0 : G(a.type)[c];
-            const R rdx = (R)a.dx, rdy = (R)a.dy;
-            const R idx = (R)a.inv_dx, idy = (R)a.inv_dy;
+            const R rdy = (R)a.dy, idy = (R)a.inv_dy;
             const double c2dt = a.c2 * dt;
             double res[4];
 #pragma unroll
@@ -1007,14 +1040,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
                                                   : G(a.in)[q * a.plane + c];
                 res[q] = u;
                 if (ty == 0) {
-                    const R fl = F[q * NF + il * TJ + jl];
-                    const R fr = F[q * NF + (il + 1) * TJ + jl];
-                    const R gb = F[q * NF + NXF + il * (TJ + 1) + jl];
-                    const R gt = F[q * NF + NXF + il * (TJ + 1) + jl + 1];
+                    const R gb = F[q * NF + YF + il * (TJ + 1) + jl];
+                    const R gt = F[q * NF + YF + il * (TJ + 1) + jl + 1];
+                    const R txq = HALF ? tx[q] : rhs_x<R, Sh, HALF>(a, F, q, il, jl);
                     // x/d == x*(1/d) bit for bit when d is a power of two
-                    const R tx = a.dx_pow2 ? (-(fr - fl)) * idx : K<R>::div(-(fr - fl), rdx);
                     const R tyy = a.dy_pow2 ? (gt - gb) * idy : K<R>::div(gt - gb, rdy);
-                    const R rhs = tx - tyy;
+                    const R rhs = txq - tyy;
                     const double L = (double)rhs;
                     if (a.form == 0) res[q] = a.c1 * u + c2dt * L;
                     else if (a.form == 1) res[q] = (a.c0 * G(a.u0)[q * a.plane + c] + a.c1 * u) + c2dt * L;
@@ -1039,13 +1070,18 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
 // Waves per SIMD the stage kernels are compiled for: the second argument of
 // __launch_bounds__ is the minimum number of waves per execution unit (SIMD)
 // on this target, which sets the VGPR budget (512 / waves).  The wanted
-// figure -- FP64 5, FP32 6, with the limiter FP64 4 (95 VGPRs), viscous FP64
+// figure -- FP64 5, FP32 6, with the limiter FP64 5, viscous FP64
 // 3 (118 VGPRs) and FP32 6 -- is capped by the waves per SIMD that the
 // workgroup's LDS (160 KB per CU) allows.
 template <typename R, bool VISC, bool LIMIT, typename Sh> struct Occupancy {
-    static constexpr int want = VISC ? (sizeof(R) == 8 ? 3 : 6) : LIMIT ? (sizeof(R) == 8 ? 4 : 6) : (sizeof(R) == 8 ? 5 : 6);
+    // the single-solve FP64 limiter kernel keeps one flux plane-set of x-faces
+    // (HALF): 30 KB instead of 38 KB of LDS per 15 x 16 tile, five workgroups
+    // per CU instead of four
+    static constexpr bool half_flux = sizeof(R) == 8 && LIMIT && !VISC;
+    static constexpr int want = VISC ? (sizeof(R) == 8 ? 3 : 6) : LIMIT ? (sizeof(R) == 8 ? 5 : 6) : (sizeof(R) == 8 ? 5 : 6);
     static constexpr int extra = VISC ? (int)sizeof(ViscLDS<R, Sh>) : LIMIT ? (int)sizeof(LimitLDS<R, Sh>) : 0;
-    static constexpr int lds = (int)sizeof(R) * (Sh::SR * Sh::SC * 4 + (Sh::NXF + Sh::NYF) * 4) + 8 * (Sh::NT / 64) + extra;
+    static constexpr int lds = (int)sizeof(R) * (Sh::SR * Sh::SC * 4 + Sh::template flux_plane<half_flux>() * 4) +
+                               8 * (Sh::NT / 64) + extra;
     static constexpr int by_lds = (160 * 1024 / lds) * (Sh::NT / 64) / 4;
     static constexpr int waves_per_simd = want < by_lds ? want : (by_lds > 0 ? by_lds : 1);
     // the batched kernel also holds its problem's arguments, which at the
@@ -1061,7 +1097,7 @@ template <typename R, bool VISC, bool LIMIT, typename Sh> struct Occupancy {
 
 // The stage kernel: one workgroup per tile.  With 15 x 16 tiles: FP64 at
 // most 96 VGPRs so that five workgroups (20 waves) share a CU; FP32 six; the
-// viscous variant's LDS (52 KB in FP64) allows three, the limiter's (38 KB) four.
+// viscous variant's LDS (52 KB in FP64) allows three, the limiter's (30 KB, HALF) five.
 //
 // Tile order: workgroups are dispatched round-robin over the 8 XCDs
 // (workgroup b to XCD b % 8), each with its own L2.  The grid is 8 x ceil(n/8)
@@ -1070,8 +1106,9 @@ template <typename R, bool VISC, bool LIMIT, typename Sh> struct Occupancy {
 // cells, are staged through the same L2.
 template <typename R, bool VISC, bool LIMIT, typename Sh>
 __global__ __launch_bounds__(Sh::NT, (Occupancy<R, VISC, LIMIT, Sh>::waves_per_simd)) void stage_kernel(StageArgs a) {
+    constexpr bool HALF = Occupancy<R, VISC, LIMIT, Sh>::half_flux;
     __shared__ R S[Sh::SR * Sh::SC * 4];
-    __shared__ R F[(Sh::NXF + Sh::NYF) * 4];
+    __shared__ R F[Sh::template flux_plane<HALF>() * 4];
     __shared__ double wmax[Sh::NT / 64];
     const int chunk = (a.ntiles + 7) >> 3;
     const int tile = (int)(blockIdx.x & 7) * chunk + (int)(blockIdx.x >> 3);
@@ -1081,7 +1118,7 @@ __global__ __launch_bounds__(Sh::NT, (Occupancy<R, VISC, LIMIT, Sh>::waves_per_s
         stage_tile<R, true, false, Sh>(a, tile, S, F, wmax, &V, nullptr);
     } else if constexpr (LIMIT) {
         __shared__ LimitLDS<R, Sh> Lm;
-        stage_tile<R, false, true, Sh>(a, tile, S, F, wmax, nullptr, &Lm);
+        stage_tile<R, false, true, Sh, true, HALF>(a, tile, S, F, wmax, nullptr, &Lm);
     } else {
         stage_tile<R, false, false, Sh>(a, tile, S, F, wmax, nullptr, nullptr);
     }
