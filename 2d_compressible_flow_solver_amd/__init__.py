@@ -35,6 +35,8 @@ CFS_E_STATE = -4
 CFS_E_NONFINITE = -5
 CFS_E_UNSUPPORTED = -6
 CFS_E_FULL = -7
+CFS_E_LIMIT = -8
+NOALLOC_MAX_CELLS = 1 << 26
 SLOT_FREE, SLOT_RUNNING, SLOT_DONE, SLOT_FAILED = 0, 1, 2, 3
 
 BC_NEUMANN, BC_DIRICHLET, BC_PERIODIC = 0, 1, 2
@@ -267,6 +269,8 @@ def _lib():
     lib.cfs_advance.argtypes = [vp, i32, dbl, dbl, dbl, P(dbl), P(i32)]
     lib.cfs_compute_dt.argtypes = [vp, dbl, P(dbl)]
     lib.cfs_checksums.argtypes = [vp, vp]
+    lib.cfs_checksums_device_noalloc.argtypes = [vp, ctypes.c_int64, vp, vp, vp]
+    lib.cfs_checksums_device_noalloc.restype = i32
     lib.cfs_stage_counter.argtypes = [vp]
     lib.cfs_stage_counter.restype = i32
     lib.cfs_last_kernel_ms.argtypes = [vp, P(dbl)]

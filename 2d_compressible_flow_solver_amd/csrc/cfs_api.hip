@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -20,6 +21,19 @@
 #include "kernels.hpp"
 
 using namespace cfs;
+
+// Device scratch of cfs_checksums_device_noalloc: the partial sums of its
+// fixed-order tree.  Level one holds 4 x ceil(cells/256) <= 4 x 2^18 sums,
+// level two 4 x ceil(cells/256^2) <= 4 x 2^10; the later levels (at most 4 x 4
+// sums, then the 4 results, written to the caller's output) reuse level one's
+// region.  Every entry a level reads was written by the level before it in the
+// same call, so nothing needs clearing between calls.  Host code reaches the
+// array only through the per-device address from hipGetSymbolAddress.
+namespace {
+constexpr long long NOALLOC_L1 = 4 * (CFS_NOALLOC_MAX_CELLS / 256);
+constexpr long long NOALLOC_L2 = 4 * (CFS_NOALLOC_MAX_CELLS / (256 * 256));
+}  // namespace
+static __device__ __align__(256) double scratch[NOALLOC_L1 + NOALLOC_L2];
 
 namespace {
 
@@ -191,6 +205,7 @@ const char* cfs_status_string(int32_t st) {
         case CFS_E_NONFINITE: return "CFS_E_NONFINITE: non-finite reduction";
         case CFS_E_UNSUPPORTED: return "CFS_E_UNSUPPORTED: configuration not supported";
         case CFS_E_FULL: return "CFS_E_FULL: no free batch slot or arena space";
+        case CFS_E_LIMIT: return "CFS_E_LIMIT: input larger than the entry point's fixed device scratch";
         default: return "unknown cfs status";
     }
 }
@@ -523,6 +538,65 @@ int32_t cfs_checksums(cfs_solver* s, double out[4]) {
     HIP_OK(hipMemcpyAsync(tmp, src, sizeof(double) * 4, hipMemcpyDeviceToHost, s->stream));
     HIP_OK(hipStreamSynchronize(s->stream));
     for (int q = 0; q < 4; ++q) out[q] = tmp[q];
+    return CFS_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex noalloc_mutex;
+double* noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The current device's address of `scratch`, looked up once per device.
+int noalloc_scratch(double** out) {
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return CFS_E_UNSUPPORTED;
+    std::lock_guard<std::mutex> lk(noalloc_mutex);
+    if (!noalloc_ptr[dev]) {
+        void* p = nullptr;
+        HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return CFS_E_HIP;
+        noalloc_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = noalloc_ptr[dev];
+    return CFS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t cfs_checksums_device_noalloc(const double* dU, int64_t cells, const uint8_t* d_type, double* d_out,
+                                     void* hip_stream) {
+    if (!dU || !d_out || cells < 1) return CFS_E_ARG;
+    if (cells > CFS_NOALLOC_MAX_CELLS) return CFS_E_LIMIT;
+    double* base = nullptr;
+    const int rc = noalloc_scratch(&base);
+    if (rc) return rc;
+    const hipStream_t stream = static_cast<hipStream_t>(hip_stream);
+    // the tree of cfs_checksums, level by level; the last level (one sum per
+    // variable) goes to d_out
+    double* bufs[2] = {base, base + NOALLOC_L1};
+    const double* src = dU;
+    long long n = cells, stride = cells;
+    const uint8_t* type = d_type;
+    int which = 0;
+    while (true) {
+        const long long nb = (n + 255) / 256;
+        double* dst = nb == 1 ? d_out : bufs[which];
+        hipLaunchKernelGGL(tree_sum_kernel, dim3((unsigned)nb, 4), dim3(256), 0, stream, src, n, stride, type, dst,
+                           nb);
+        HIP_OK(hipGetLastError());
+        if (nb == 1) break;
+        src = dst;
+        stride = nb;
+        n = nb;
+        type = nullptr;
+        which ^= 1;
+    }
     return CFS_OK;
 }
 

@@ -39,7 +39,8 @@ typedef enum cfs_status {
     CFS_E_STATE = -4,       /* call out of sequence (e.g. stage counter mismatch) */
     CFS_E_NONFINITE = -5,   /* a reduction met NaN/Inf (e.g. negative pressure) */
     CFS_E_UNSUPPORTED = -6, /* configuration not supported by this build */
-    CFS_E_FULL = -7         /* batch: no free slot or arena space for the problem now */
+    CFS_E_FULL = -7,        /* batch: no free slot or arena space for the problem now */
+    CFS_E_LIMIT = -8        /* input larger than the fixed device scratch of a no-allocation entry point */
 } cfs_status;
 
 /* Physical boundary kinds.  NEUMANN and DIRICHLET follow
@@ -205,6 +206,25 @@ int32_t cfs_advance(cfs_solver* s, int32_t nsteps, double dt, double cfl,
 int32_t cfs_compute_dt(cfs_solver* s, double cfl, double* dt_out);
 /* Fixed-order FP64 tree sums of (rho, rho*u, rho*v, E) over fluid cells. */
 int32_t cfs_checksums(cfs_solver* s, double out[4]);
+/* cfs_checksums of a state that the caller holds in device memory, with no
+ * handle and no device allocation: the tree's partial sums live in a fixed
+ * file-scope device array of the library (about 8.3 MB per GPU, held until
+ * the process ends).  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops
+ * that check conservation totals of resident states).
+ *   dU     [4][cells] FP64 state on the current device (x-major planes)
+ *   d_type [cells] cell types (CFS_CELL_*; only fluid cells are summed), or NULL
+ *   d_out  4 doubles of device memory that receive the sums of
+ *          (rho, rho*u, rho*v, E), the same bits as cfs_checksums
+ *   hip_stream  stream the kernels are queued on (NULL: the null stream);
+ *          the call returns without waiting.
+ * cells must lie in [1, CFS_NOALLOC_MAX_CELLS]: larger inputs are CFS_E_LIMIT
+ * before anything is queued.  Every call on one GPU shares that GPU's scratch
+ * array: calls whose work overlaps on one GPU (other streams, other threads)
+ * race and must be ordered by the caller. */
+#define CFS_NOALLOC_MAX_CELLS (1ll << 26)
+int32_t cfs_checksums_device_noalloc(const double* dU, int64_t cells, const uint8_t* d_type, double* d_out,
+                                     void* hip_stream);
 /* Copies the current state plane-set into a spare one `repeats` times with a
  * plain streaming copy kernel (32*nx*ny bytes read and written per launch).
  * No reference counterpart: it calibrates hardware byte counters against a

@@ -30,7 +30,7 @@ def test_version_and_layout():
     assert lib.cfs_abi_version() == int(re.search(r"#define CFS_ABI_VERSION (\d+)", open(cfs.HEADER_PATH).read()).group(1))
     lib.cfs_config_size.restype = ctypes.c_int32
     assert lib.cfs_config_size() == ctypes.sizeof(cfs.Config)
-    for st in range(-7, 1):
+    for st in range(-8, 1):
         assert b"unknown" not in lib.cfs_status_string(st)
     assert cfs.device_count() >= 0
 
@@ -113,3 +113,13 @@ def test_tile_shapes_without_a_gpu():
     bc = cfs.BatchConfig(3, cfs.PREC_FP64, 4, 0, 1 << 20, 0, cfs.NUM_TILE_SHAPES)
     assert lib.cfs_batch_create(ctypes.byref(bc), None, ctypes.byref(h)) == cfs.CFS_E_ARG and not h.value
     assert lib.cfs_batch_autotune(ctypes.byref(bc), None, 0, 5, None, None) == cfs.CFS_E_ARG
+
+
+def test_noalloc_checksums_refuse_bad_sizes_without_a_gpu():
+    # both refusals come before any HIP call; the pointers are never read
+    lib = cfs.library()
+    fake = ctypes.c_void_p(256)
+    assert lib.cfs_checksums_device_noalloc(fake, cfs.NOALLOC_MAX_CELLS + 1, None, fake, None) == cfs.CFS_E_LIMIT
+    assert lib.cfs_checksums_device_noalloc(fake, 0, None, fake, None) == cfs.CFS_E_ARG
+    assert lib.cfs_checksums_device_noalloc(None, 16, None, fake, None) == cfs.CFS_E_ARG
+    assert b"CFS_E_LIMIT" in lib.cfs_status_string(cfs.CFS_E_LIMIT)
