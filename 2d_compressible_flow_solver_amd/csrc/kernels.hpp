@@ -751,7 +751,10 @@ __device__ __forceinline__ void viscous_flux(const ViscLDS<R, Sh>& V, const int 
 // DIR = 1 or 2 fixes the face's direction at compile time (its LDS strides and
 // flux components become constants); DIR = 0 decides it from s at run time
 // (one body for both directions: fewer registers, more index arithmetic).
-template <typename R, bool VISC, bool LIMIT, bool FAST, typename Sh, int DIR, bool HALF>
+// REG: no face of this direction in the tile is a degenerate boundary face
+// (0 or n of a non-periodic axis), so every stencil is the plain 6 cells and
+// its LDS offsets are constants.
+template <typename R, bool VISC, bool LIMIT, bool FAST, typename Sh, int DIR, bool HALF, bool REG = false>
 __device__ __forceinline__ void tile_face(const StageArgs& a, int s, int i0, int j0, R g, R lam_x, R lam_y,
                                           const R* S, R* F, const ViscLDS<R, Sh>* V, const LimitLDS<R, Sh>* Lm) {
     constexpr int TJ = Sh::TJ, SC = Sh::SC, NXF = Sh::NXF;
@@ -768,8 +771,8 @@ __device__ __forceinline__ void tile_face(const StageArgs& a, int s, int i0, int
             int o = k - 3;
             // boundary faces: the walk stops at the boundary face, so the
             // stencil is [G,G,G,c0,c0,c0] at f = 0 and [c,c,c,G,G,G] at f = n
-            if (!px && f == 0 && o > 0) o = 0;
-            if (!px && f == a.nx && o < -1) o = -1;
+            if (!REG && !px && f == 0 && o > 0) o = 0;
+            if (!REG && !px && f == a.nx && o < -1) o = -1;
             cell[k] = (fr + o + HALO) * SC + jl + HALO;
             if (VISC && k >= 1 && k <= 4) gcell[k - 1] = (fr + o + HALO - 1) * ViscLDS<R, Sh>::GYW + jl;
         }
@@ -781,8 +784,8 @@ __device__ __forceinline__ void tile_face(const StageArgs& a, int s, int i0, int
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
             int o = k - 3;
-            if (!py && gf == 0 && o > 0) o = 0;
-            if (!py && gf == a.ny && o < -1) o = -1;
+            if (!REG && !py && gf == 0 && o > 0) o = 0;
+            if (!REG && !py && gf == a.ny && o < -1) o = -1;
             cell[k] = (il + HALO) * SC + gl + o + HALO;
             if (VISC && k >= 1 && k <= 4) gcell[k - 1] = il * ViscLDS<R, Sh>::GXW + gl + o + HALO - 1;
         }
@@ -835,11 +838,22 @@ __device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, d
     constexpr int NXF = Sh::NXF, NF = Sh::NXF + Sh::NYF;
     const int tid = threadIdx.x;
     const R g = (R)a.gamma;
-    // Lax-Friedrichs shares of the limiter: dt/dx and dt/dy
-    const R lam_x = (R)(dt / a.dx), lam_y = (R)(dt / a.dy);
+    // Lax-Friedrichs shares of the limiter: dt/dx and dt/dy (x/d == x*(1/d)
+    // bit for bit when d is a power of two)
+    const R lam_x = (R)(a.dx_pow2 ? dt * a.inv_dx : dt / a.dx);
+    const R lam_y = (R)(a.dy_pow2 ? dt * a.inv_dy : dt / a.dy);
     if constexpr (SPLIT) {
-        for (int s = tid; s < NXF; s += Sh::NT)
-            tile_face<R, VISC, LIMIT, FAST, Sh, 1, HALF>(a, s, i0, j0, g, lam_x, lam_y, S, F, V, Lm);
+        // a direction whose tile faces include no degenerate boundary face
+        // (the fast division forms only: the IEEE form serves rare tiles)
+        const bool xreg = FAST && (a.bc[0] == BC_PERIODIC || (i0 > 0 && i0 + Sh::TI < a.nx));
+        const bool yreg = FAST && (a.bc[2] == BC_PERIODIC || (j0 > 0 && j0 + Sh::TJ < a.ny));
+        if (xreg) {
+            for (int s = tid; s < NXF; s += Sh::NT)
+                tile_face<R, VISC, LIMIT, FAST, Sh, 1, HALF, FAST>(a, s, i0, j0, g, lam_x, lam_y, S, F, V, Lm);
+        } else {
+            for (int s = tid; s < NXF; s += Sh::NT)
+                tile_face<R, VISC, LIMIT, FAST, Sh, 1, HALF>(a, s, i0, j0, g, lam_x, lam_y, S, F, V, Lm);
+        }
         if constexpr (HALF) {
             __syncthreads();
             if (tid < Sh::TI * Sh::TJ) {
@@ -849,8 +863,13 @@ __device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, d
             }
             __syncthreads();
         }
-        for (int s = NXF + tid; s < NF; s += Sh::NT)
-            tile_face<R, VISC, LIMIT, FAST, Sh, 2, HALF>(a, s, i0, j0, g, lam_x, lam_y, S, F, V, Lm);
+        if (yreg) {
+            for (int s = NXF + tid; s < NF; s += Sh::NT)
+                tile_face<R, VISC, LIMIT, FAST, Sh, 2, HALF, FAST>(a, s, i0, j0, g, lam_x, lam_y, S, F, V, Lm);
+        } else {
+            for (int s = NXF + tid; s < NF; s += Sh::NT)
+                tile_face<R, VISC, LIMIT, FAST, Sh, 2, HALF>(a, s, i0, j0, g, lam_x, lam_y, S, F, V, Lm);
+        }
     } else {
         for (int s = tid; s < NF; s += Sh::NT)
             tile_face<R, VISC, LIMIT, FAST, Sh, 0, HALF>(a, s, i0, j0, g, lam_x, lam_y, S, F, V, Lm);
