@@ -551,6 +551,7 @@ __device__ __forceinline__ void block_max_to(double m, unsigned long long* acc, 
 
 template <typename R, typename Sh> struct LimitLDS {
     R sx[Sh::SR * Sh::SC], sy[Sh::SR * Sh::SC];   // |u| + c and |v| + c of every staged cell
+    R ir[Sh::SR * Sh::SC], p[Sh::SR * Sh::SC];    // 1/rho and the pressure of every staged cell
 };
 
 template <typename R> __device__ __forceinline__ R maxr(R x, R y) { return (x < y) ? y : x; }
@@ -609,14 +610,15 @@ __device__ __forceinline__ R share_theta(const R B0[4], const R Dl[4], R gm1) {
     }
 }
 
-// Euler flux of a cell state in the face-normal frame, one division: with
-// ir = 1/r, un = n*ir, ut = t*ir, p = gm1*(E - half*(n*un + t*ut))
+// Euler flux of a cell state in the face-normal frame from the cell's 1/rho
+// (ir) and pressure p, both computed once per staged cell: un = n*ir,
+// ut = t*ir, p = gm1*(E - half*(rho u^2 + rho v^2)) with the two momentum
+// terms in either order (IEEE addition commutes, so both directions see the
+// same bits)
 template <typename R>
-__device__ __forceinline__ void euler_flux_n(R r, R n, R t, R E, R gm1, R F[4]) {
+__device__ __forceinline__ void euler_flux_n(R n, R t, R E, R ir, R p, R F[4]) {
     using C = K<R>;
-    const R ir = C::div(R(1), r);
     const R un = n * ir, ut = t * ir;
-    const R p = gm1 * (E - C::half * C::madd(n, un, t * ut));
     F[0] = n;
     F[1] = C::madd(n, un, p);
     F[2] = n * ut;
@@ -635,8 +637,8 @@ __device__ __forceinline__ void limit_face(const R* __restrict__ S, int ns, cons
     const R UL[4] = {S[cL], S[nrm * ns + cL], S[tng * ns + cL], S[3 * ns + cL]};
     const R UR[4] = {S[cR], S[nrm * ns + cR], S[tng * ns + cR], S[3 * ns + cR]};
     R FL[4], FR[4];
-    euler_flux_n<R>(UL[0], UL[1], UL[2], UL[3], gm1, FL);
-    euler_flux_n<R>(UR[0], UR[1], UR[2], UR[3], gm1, FR);
+    euler_flux_n<R>(UL[1], UL[2], UL[3], Lm.ir[cL], Lm.p[cL], FL);
+    euler_flux_n<R>(UR[1], UR[2], UR[3], Lm.ir[cR], Lm.p[cR], FR);
     const R a = maxr<R>(sn[cL], sn[cR]);
     R Flf[4], D[4];
 #pragma unroll
@@ -983,6 +985,8 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
                 const R c = K<R>::sqrt_((g * p) * ir);
                 Lm->sx[s] = (ux < R(0) ? -ux : ux) + c;
                 Lm->sy[s] = (uy < R(0) ? -uy : uy) + c;
+                Lm->ir[s] = ir;
+                Lm->p[s] = p;
             }
             __syncthreads();
         }
