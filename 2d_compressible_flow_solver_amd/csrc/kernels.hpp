@@ -508,6 +508,27 @@ __device__ __forceinline__ double cell_speed(double rho, double mu, double mv, d
     return (ax + cs) / dx + (ay + cs) / dy;
 }
 
+// The same bound for FP32-bulk stages, still in FP64 but with one refined
+// reciprocal of rho (two Newton steps after the hardware reciprocal) and
+// multiplications by 1/dx, 1/dy in place of the five IEEE divisions: within
+// a few ulp of cell_speed.  A non-positive density or pressure gives a
+// non-finite bound, as there.
+__device__ __forceinline__ double cell_speed_bulk(double rho, double mu, double mv, double E, double gamma,
+                                                  double inv_dx, double inv_dy, int hll_bounds) {
+    double ir = __builtin_amdgcn_rcp(rho);
+    ir = __builtin_fma(ir, __builtin_fma(-rho, ir, 1.0), ir);
+    ir = __builtin_fma(ir, __builtin_fma(-rho, ir, 1.0), ir);
+    const double u = mu * ir, v = mv * ir;
+    const double p = (gamma - 1) * (E - 0.5 * (mu * u + mv * v));
+    const double cs = __builtin_sqrt((gamma * p) * ir);
+    double ax = __builtin_fabs(u), ay = __builtin_fabs(v);
+    if (hll_bounds == 0) {
+        if (ax < __builtin_fabs(mu)) ax = __builtin_fabs(mu);
+        if (ay < __builtin_fabs(mv)) ay = __builtin_fabs(mv);
+    }
+    return (ax + cs) * inv_dx + (ay + cs) * inv_dy;
+}
+
 __device__ __forceinline__ double max_nan(double a, double b) { return (b > a || b != b) ? b : a; }
 
 // Block-wide max (NaN wins) -> atomicMax on the bit pattern.  The atomic is
@@ -1084,7 +1105,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
                 for (int q = 0; q < 4; ++q) G(a.out)[q * a.plane + c] = res[q];
             }
             if (a.fuse_cfl && ty == 0)
-                smax = cell_speed(res[0], res[1], res[2], res[3], a.gamma, a.dx, a.dy, a.hll_bounds);
+                smax = (sizeof(R) == 8)
+                           ? cell_speed(res[0], res[1], res[2], res[3], a.gamma, a.dx, a.dy, a.hll_bounds)
+                           : cell_speed_bulk(res[0], res[1], res[2], res[3], a.gamma, a.inv_dx, a.inv_dy, a.hll_bounds);
         }
     }
     if (a.fuse_cfl) block_max_to(smax, a.acc_out, wmax);
