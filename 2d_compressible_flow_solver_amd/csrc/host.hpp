@@ -265,7 +265,7 @@ inline int launch_cfl(const ProblemDev& p, int k, unsigned long long* acc, hipSt
     const cfs_config& c = *p.cfg;
     hipLaunchKernelGGL(set_u64_kernel, dim3(1), dim3(1), 0, stream, acc, 0ull);
     hipLaunchKernelGGL(cfl_kernel, dim3(2048), dim3(256), 0, stream, p.set(k), p.plane, p.type, c.gamma,
-                       c.lx / c.nx, c.ly / c.ny, c.hll_bounds, acc);
+                       c.lx / c.nx, c.ly / c.ny, c.hll_bounds, c.precision == CFS_PREC_FP32_BULK ? 1 : 0, acc);
     HIP_OK(hipGetLastError());
     return CFS_OK;
 }

@@ -1234,17 +1234,24 @@ static __global__ __launch_bounds__(256) void ib_fill_kernel(IBArgs a) {
 // CFL reduction over a whole state (first step of an advance, cfs_compute_dt):
 // wave64 shuffles, LDS, one atomicMax on the bit pattern (non-negative doubles
 // order as unsigned integers; a NaN wins) -- exact, so order-free.
+// bulk: the FP32-bulk form of the bound (cell_speed_bulk), the one its
+// stages fold into their last sub-stage, so that an advance that starts from
+// a stored state takes the steps an uninterrupted one takes.
 static __global__ __launch_bounds__(256) void cfl_kernel(const double* __restrict__ U, long long plane,
                                                   const uint8_t* __restrict__ type, double gamma,
-                                                  double dx, double dy, int hll_bounds,
+                                                  double dx, double dy, int hll_bounds, int bulk,
                                                   unsigned long long* acc) {
     __shared__ double wmax[4];
     double m = 0.0;
+    const double inv_dx = 1.0 / dx, inv_dy = 1.0 / dy;
     for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < plane;
          c += (long long)gridDim.x * blockDim.x) {
         if (type && type[c] != 0) continue;
-        m = max_nan(m, cell_speed(U[c], U[plane + c], U[2 * plane + c], U[3 * plane + c], gamma, dx, dy,
-                                  hll_bounds));
+        const double s = bulk ? cell_speed_bulk(U[c], U[plane + c], U[2 * plane + c], U[3 * plane + c], gamma,
+                                                inv_dx, inv_dy, hll_bounds)
+                              : cell_speed(U[c], U[plane + c], U[2 * plane + c], U[3 * plane + c], gamma, dx, dy,
+                                           hll_bounds);
+        m = max_nan(m, s);
     }
     block_max_to(m, acc, wmax);
 }

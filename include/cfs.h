@@ -195,7 +195,9 @@ int32_t cfs_step(cfs_solver* s, double dt);
  * a_x = max(|u|,|rho*u|) with the reference HLL bounds (hll_bounds = 0: they
  * use the momentum where the velocity belongs, RiemmanSolverHLL.cpp:41-42),
  * a_x = |u| with velocity bounds; likewise a_y.  Reduced on the device (no
- * reference counterpart: the reference uses a fixed dt, param.h:20).
+ * reference counterpart: the reference uses a fixed dt, param.h:20).  FP64
+ * evaluates it with IEEE divisions; FP32-bulk, still in FP64, with one refined
+ * reciprocal of rho and multiplications by 1/dx, 1/dy (within a few ulp).
  * t_end > 0 clips the last step so that the time does not pass t_end and stops there.
  * On return *t_io is advanced by the simulated time (may be NULL). */
 int32_t cfs_advance(cfs_solver* s, int32_t nsteps, double dt, double cfl,

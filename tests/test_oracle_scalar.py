@@ -192,6 +192,42 @@ def test_hll_uses_momentum_in_outer_bounds():
     del R
 
 
+def test_hll_right_bound_takes_the_left_state():
+    # Known answer for RiemmanSolverHLL.cpp:41-42, where the outer right bound
+    # is SR = max(m_L + a_L, u* + a*_R): the LEFT state's momentum and sound
+    # speed, where the textbook bound takes the right state's.  Hand
+    # derivation, rho = 1 on both sides (momentum = velocity), p = 2, so
+    # a = sqrt(1.4 * 2) = 1.67332; u_L = -2, u_R = -1.5:
+    #   PVRS: u* = (u_L + u_R)/2 = -1.75 (equal pressures),
+    #         p* = 2 - (u_R - u_L)/2 * rho * a = 1.58167 > 0,
+    #         rho*_R = 1 - (u_R - u*) rho / a = 0.85060,
+    #         a*_R = sqrt(1.4 p* / rho*_R) = 1.61347;
+    #   u_L + a_L = -0.32668 and u* + a*_R = -0.13653: SR = -0.13653 <= 0, so
+    #   the right state's upwind flux F(R) is taken, bit for bit;
+    #   with the right state in SR, u_R + a_R = +0.17332 > 0 would select the
+    #   HLL average instead (whose mass flux differs from m_R by ~0.03).
+    a = (1.4 * 2.0) ** 0.5
+    uL, uR = -2.0, -1.5
+    ps = 2.0 - 0.5 * (uR - uL) * a
+    rs = 1.0 - (uR - (uL + uR) / 2) / a
+    asR = (1.4 * ps / rs) ** 0.5
+    assert uL + a < 0 and (uL + uR) / 2 + asR < 0 < uR + a and ps > 0
+    L, R = _cons(1.0, uL, 0.0, 2.0), _cons(1.0, uR, 0.0, 2.0)
+    f = O.hll(G, L, R, True)
+    FR = O.euler_flux(G, R, True)
+    assert np.array_equal(f, FR)
+    # the HLL average of the textbook bounds for comparison: SL from the same
+    # formula, SR = u_R + a_R
+    SL = min(uL - a, (uL + uR) / 2 - (1.4 * ps / (1.0 + (uL - (uL + uR) / 2) / a)) ** 0.5)
+    SR = uR + a
+    FL = O.euler_flux(G, L, True)
+    h0 = (SR * FL[0] - SL * FR[0] + SL * SR * (R[0] - L[0])) / (SR - SL)
+    assert abs(h0 - FR[0]) > 0.02
+    # the same pair across a y-face (momentum components swapped)
+    Ly, Ry = [L[0], L[2], L[1], L[3]], [R[0], R[2], R[1], R[3]]
+    assert np.array_equal(O.hll(G, Ly, Ry, False), O.euler_flux(G, Ry, False))
+
+
 def test_hll_x_y_mirror():
     # the y flux of a state equals the x flux of the state with its momentum
     # components swapped, components swapped back (bit for bit)
