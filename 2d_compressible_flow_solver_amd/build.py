@@ -66,7 +66,7 @@ def build(verbose=False):
     tmp = RUN + ".tmp"
     _run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-Wall", "-o", tmp,
           os.path.join(drv, "run.cpp"), os.path.join(drv, "main.cpp"),
-          "-L" + HERE, "-lcfs", "-Wl,-rpath,$ORIGIN"], verbose)
+          "-L" + HERE, "-lcfs", "-ldl", "-Wl,-rpath,$ORIGIN"], verbose)
     os.replace(tmp, RUN)
     return LIB
 

@@ -1,9 +1,11 @@
 // output.h -- snapshots in the reference's field layout (Output::WriteToHDF5,
 // Output.cpp:32-88): datasets Mu, Mv, Rho, Energy, Pressure, FluxMu, FluxMv,
-// FluxRho, FluxEnergy, Udir, Vdir, Tdir, each [Nx][Ny] x-major FP64.  No HDF5
-// library is available here, so each dataset is a NumPy .npy file (a short
-// text header, then the raw little-endian doubles) in one directory per
-// snapshot, with meta.json naming the stage, time and sizes.
+// FluxRho, FluxEnergy, Udir, Vdir, Tdir, each [Nx][Ny] x-major FP64.  One
+// directory per snapshot holds them as snapshot.h5 (serial HDF5, the
+// reference's dataset names in the root group) when an HDF5 C library is
+// found at run time (hdf5.h), and always as one NumPy .npy file per dataset (a
+// short text header, then the raw little-endian doubles), with meta.json
+// naming the stage, time, sizes and the HDF5 file if one was written.
 #pragma once
 
 #include <cstdint>
@@ -12,6 +14,7 @@
 #include <vector>
 
 #include "../../include/cfs.h"
+#include "hdf5.h"
 
 namespace cfsdrv {
 
@@ -62,13 +65,20 @@ public:
                               {"Udir", Z.data()},    {"Vdir", Z.data()},     {"Tdir", Z.data()}};
         for (const Item& it : items)
             if (!write_npy(dir + "/" + it.name + ".npy", it.p, cfg.nx, cfg.ny)) return CFS_E_ARG;
+        std::string h5 = "null";
+        if (const Hdf5* lib = Hdf5::get()) {
+            std::vector<Hdf5::Field> fields;
+            for (const Item& it : items) fields.push_back(Hdf5::Field{it.name, it.p});
+            if (!lib->write(dir + "/snapshot.h5", cfg.nx, cfg.ny, fields)) return CFS_E_ARG;
+            h5 = "\"snapshot.h5\"";
+        }
         FILE* f = std::fopen((dir + "/meta.json").c_str(), "w");
         if (!f) return CFS_E_ARG;
         std::fprintf(f,
                      "{\"stage\": %d, \"t\": %.17g, \"nx\": %d, \"ny\": %d, \"layout\": \"[nx][ny] x-major float64\", "
                      "\"datasets\": [\"Mu\", \"Mv\", \"Rho\", \"Energy\", \"Pressure\", \"FluxMu\", \"FluxMv\", "
-                     "\"FluxRho\", \"FluxEnergy\", \"Udir\", \"Vdir\", \"Tdir\"]}\n",
-                     stage, t, cfg.nx, cfg.ny);
+                     "\"FluxRho\", \"FluxEnergy\", \"Udir\", \"Vdir\", \"Tdir\"], \"hdf5\": %s}\n",
+                     stage, t, cfg.nx, cfg.ny, h5.c_str());
         std::fclose(f);
         return CFS_OK;
     }

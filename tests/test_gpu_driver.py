@@ -16,10 +16,23 @@ pytestmark = pytest.mark.gpu
 RUN = os.path.join(os.path.dirname(cfs.LIB_PATH), "cfs_run")
 
 
+NAMES = ("Mu", "Mv", "Rho", "Energy", "Pressure", "FluxMu", "FluxMv", "FluxRho", "FluxEnergy", "Udir", "Vdir",
+         "Tdir")
+
+
 def _load(d):
-    return {k: np.load(os.path.join(d, k + ".npy")) for k in
-            ("Mu", "Mv", "Rho", "Energy", "Pressure", "FluxMu", "FluxMv", "FluxRho", "FluxEnergy",
-             "Udir", "Vdir", "Tdir")}
+    """The snapshot's .npy datasets; where an HDF5 library is present, its
+    snapshot.h5 must hold the same datasets (the reference's names, [nx][ny]
+    native double) with the same bits."""
+    out = {k: np.load(os.path.join(d, k + ".npy")) for k in NAMES}
+    import _h5
+    meta = json.load(open(os.path.join(d, "meta.json")))
+    if _h5.lib() is not None:
+        assert meta["hdf5"] == "snapshot.h5"
+        h5 = _h5.read(os.path.join(d, "snapshot.h5"), NAMES)
+        for k in NAMES:
+            assert h5[k] is not None and np.array_equal(h5[k].view(np.uint64), out[k].view(np.uint64)), k
+    return out
 
 
 def test_driver_lax_liu_snapshots(tmp_path):
