@@ -277,6 +277,14 @@ int64_t batch_problem_tiles(const cfs_batch* b, const cfs_config* cfg) { return 
 
 int64_t batch_resident_workgroups(const cfs_batch* b) { return b->resident; }
 
+int32_t batch_device_free_bytes(const cfs_batch* b, int64_t* bytes) {
+    if (hipSetDevice(b->device) != hipSuccess) return CFS_E_HIP;
+    size_t free_b = 0, total = 0;
+    if (hipMemGetInfo(&free_b, &total) != hipSuccess) return CFS_E_HIP;
+    *bytes = (int64_t)free_b;
+    return CFS_OK;
+}
+
 int32_t batch_make_device_current(const cfs_batch* b) {
     return hipSetDevice(b->device) == hipSuccess ? CFS_OK : CFS_E_HIP;
 }

@@ -18,7 +18,12 @@
 // first reads them from device memory), and the request no longer waits for
 // every other resident problem at each of its steps, which matters because
 // the largest requests take the most steps.  The arithmetic is the same
-// kernel's either way, so its result does not depend on the lane.
+// kernel's either way, so its result does not depend on the lane.  The solo
+// lane is admitted by footprint too: a worker takes the next request only
+// when its footprint fits a budget of nine tenths of the device memory free
+// at the start beside what the other worker holds, and keeps its solver
+// handle (no allocation on the request path) while the next request has the
+// same configuration.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -43,23 +48,63 @@ double seconds_since(Clock::time_point t0) {
 
 int64_t footprint_of(const cfs_request& r) { return cfs_batch_footprint(r.cfg, r.ib ? r.ib->n_ig : 0); }
 
-// One request solved alone to its end time; CFS_OK with r.state DONE, or
-// FAILED on a non-finite CFL bound (as a batch slot would fail), else the
-// error of the entry point that failed.
-int32_t solve_alone(cfs_request& r) {
+// A worker's solver handle, kept from one request to the next when the next
+// one has the same configuration (a new handle otherwise, or when the kept
+// one carries an immersed boundary the next request does not have).  `bytes`
+// is the device memory it counts against the solo lane's budget.
+struct Handle {
     cfs_solver* h = nullptr;
-    int32_t rc = cfs_create(r.cfg, nullptr, &h);
-    if (rc) return rc;
-    if (r.ib)
-        rc = cfs_set_immersed(h, r.ib->interp, r.ib->cell_type, r.ib->n_ig, r.ib->ig_cells, r.ib->donors,
+    cfs_config cfg{};
+    bool ib = false;
+    int64_t bytes = 0;
+    void drop() {
+        cfs_destroy(h);
+        h = nullptr;
+        ib = false;
+        bytes = 0;
+    }
+};
+
+bool same_config(const cfs_config& a, const cfs_config& b) {
+    if (a.nx != b.nx || a.ny != b.ny || a.lx != b.lx || a.ly != b.ly || a.x0 != b.x0 || a.y0 != b.y0 ||
+        a.gamma != b.gamma || a.rk != b.rk || a.precision != b.precision || a.viscous != b.viscous ||
+        a.positivity != b.positivity || a.hll_bounds != b.hll_bounds || a.mu != b.mu || a.prandtl != b.prandtl ||
+        a.ma_ref != b.ma_ref)
+        return false;
+    for (int k = 0; k < 4; ++k) {
+        if (a.bc[k] != b.bc[k]) return false;
+        for (int q = 0; q < 4; ++q)
+            if (a.bc_state[k][q] != b.bc_state[k][q]) return false;
+    }
+    return true;
+}
+
+bool reusable(const Handle& hd, const cfs_request& r) { return hd.h && same_config(hd.cfg, *r.cfg) && (!hd.ib || r.ib); }
+
+// One request solved alone to its end time on the worker's handle; CFS_OK
+// with r.state DONE, or FAILED on a non-finite CFL bound (as a batch slot
+// would fail), else the error of the entry point that failed.
+int32_t solve_alone(Handle& hd, cfs_request& r, int64_t footprint) {
+    int32_t rc = CFS_OK;
+    if (!reusable(hd, r)) {
+        hd.drop();
+        rc = cfs_create(r.cfg, nullptr, &hd.h);
+        if (rc) return rc;
+        hd.cfg = *r.cfg;
+        hd.bytes = footprint;
+    }
+    if (r.ib) {
+        rc = cfs_set_immersed(hd.h, r.ib->interp, r.ib->cell_type, r.ib->n_ig, r.ib->ig_cells, r.ib->donors,
                               r.ib->image_xy, r.ib->coef, r.ib->transp);
-    if (!rc) rc = cfs_set_state(h, r.U);
+        hd.ib = true;
+    }
+    if (!rc) rc = cfs_set_state(hd.h, r.U);
     double t = 0.0;
     int32_t steps = 0;
     r.state = CFS_SLOT_DONE;
     while (!rc && t < r.t_end) {
         int32_t done = 0;
-        const int32_t st = cfs_advance(h, 1 << 30, 0.0, r.cfl, r.t_end, &t, &done);
+        const int32_t st = cfs_advance(hd.h, 1 << 30, 0.0, r.cfl, r.t_end, &t, &done);
         steps += done;
         if (st == CFS_E_NONFINITE) {
             r.state = CFS_SLOT_FAILED;
@@ -68,13 +113,18 @@ int32_t solve_alone(cfs_request& r) {
         rc = st;
         if (done == 0) break;
     }
-    if (!rc && r.U_out) rc = cfs_get_state(h, r.U_out);
+    if (!rc && r.U_out) rc = cfs_get_state(hd.h, r.U_out);
     r.t = t;
     r.steps = steps;
-    cfs_destroy(h);
+    if (rc) hd.drop();
     return rc;
 }
 
+// The solo lane: requests in arrival order, each taken by a worker once its
+// footprint fits the lane's device-memory budget beside the handles the other
+// workers hold.  A worker keeps its handle after a request only while the
+// next queued request could reuse it; otherwise it frees it, so that waiting
+// requests are never blocked by idle handles.
 struct SoloLane {
     std::mutex m;
     std::condition_variable cv;
@@ -82,35 +132,55 @@ struct SoloLane {
     bool closing = false;
     int32_t error = CFS_OK;      // first failure of an entry point (under m)
     std::atomic<int> finished{0};
+    int64_t budget = 0;          // device bytes the lane's handles may hold at once
+    const std::vector<int64_t>* fp = nullptr;
+    std::vector<Handle> handles;
 
-    void run(const cfs_batch* b, cfs_request* reqs, Clock::time_point t0) {
+    int64_t held_by_others(int w) const {
+        int64_t s = 0;
+        for (int k = 0; k < (int)handles.size(); ++k)
+            if (k != w) s += handles[k].bytes;
+        return s;
+    }
+
+    void run(int w, const cfs_batch* b, cfs_request* reqs, Clock::time_point t0) {
         const int32_t dev_rc = batch_make_device_current(b);
+        Handle& hd = handles[w];
         for (;;) {
             int k;
             {
                 std::unique_lock<std::mutex> lk(m);
-                cv.wait(lk, [&] { return closing || !queue.empty(); });
-                if (queue.empty()) return;
+                cv.wait(lk, [&] {
+                    return closing || (!queue.empty() && held_by_others(w) + (*fp)[queue.front()] <= budget);
+                });
+                if (closing && queue.empty()) break;
                 k = queue.front();
                 queue.pop_front();
+                // the handle is replaced unless the request can reuse it
+                if (!reusable(hd, reqs[k])) hd.drop();
             }
+            cv.notify_all();
             cfs_request& r = reqs[k];
             r.admitted_s = seconds_since(t0);
-            const int32_t rc = dev_rc ? dev_rc : solve_alone(r);
+            const int32_t rc = dev_rc ? dev_rc : solve_alone(hd, r, (*fp)[k]);
             r.finished_s = seconds_since(t0);
-            if (rc) {
+            {
                 std::lock_guard<std::mutex> lk(m);
-                if (!error) error = rc;
+                if (rc && !error) error = rc;
+                if (queue.empty() || !reusable(hd, reqs[queue.front()])) hd.drop();
             }
             finished.fetch_add(1);
+            cv.notify_all();
         }
+        std::lock_guard<std::mutex> lk(m);
+        hd.drop();
     }
     void push(int k) {
         {
             std::lock_guard<std::mutex> lk(m);
             queue.push_back(k);
         }
-        cv.notify_one();
+        cv.notify_all();
     }
     void close() {   // the workers end after the solves in hand (nothing is queued on success)
         {
@@ -135,6 +205,11 @@ extern "C" int32_t cfs_serve(cfs_batch* b, cfs_request* reqs, int32_t n, int32_t
     int rc = cfs_batch_capacity(b, &arena, &nslots);
     if (rc) return rc;
     const int64_t resident_wg = batch_resident_workgroups(b);
+    // the solo lane's budget: nine tenths of the device memory free now (the
+    // batch's arena is already allocated)
+    int64_t free_dev = 0;
+    if ((rc = batch_device_free_bytes(b, &free_dev))) return rc;
+    const int64_t solo_budget = free_dev / 10 * 9;
     std::vector<int64_t> fp((size_t)n);
     std::vector<char> solo((size_t)n, 0);
     for (int k = 0; k < n; ++k) {
@@ -145,6 +220,7 @@ extern "C" int32_t cfs_serve(cfs_batch* b, cfs_request* reqs, int32_t n, int32_t
         if (fp[k] < 0) return CFS_E_ARG;
         solo[k] = batch_problem_tiles(b, r.cfg) >= resident_wg;
         if (!solo[k] && fp[k] > arena) return CFS_E_FULL;
+        if (solo[k] && fp[k] > solo_budget) return CFS_E_FULL;
         r.admitted_s = r.finished_s = r.t = 0.0;
         r.steps = 0;
         r.state = CFS_SLOT_FREE;
@@ -160,6 +236,9 @@ extern "C" int32_t cfs_serve(cfs_batch* b, cfs_request* reqs, int32_t n, int32_t
     int resident = 0, finished = 0;
     const Clock::time_point t0 = Clock::now();
     SoloLane lane;
+    lane.budget = solo_budget;
+    lane.fp = &fp;
+    lane.handles.resize(kSoloWorkers);
     std::vector<std::thread> workers;
     auto stop = [&](int32_t code) {
         lane.close();
@@ -167,7 +246,7 @@ extern "C" int32_t cfs_serve(cfs_batch* b, cfs_request* reqs, int32_t n, int32_t
         return code;
     };
     try {
-        for (int w = 0; w < kSoloWorkers; ++w) workers.emplace_back([&] { lane.run(b, reqs, t0); });
+        for (int w = 0; w < kSoloWorkers; ++w) workers.emplace_back([&, w] { lane.run(w, b, reqs, t0); });
     } catch (...) {
         return stop(CFS_E_OOM);
     }

@@ -17,6 +17,8 @@ CFS_INTERNAL int32_t batch_check_problem(const cfs_batch* b, const cfs_config* c
 // workgroups the GPU holds at once for the batch's stage kernel.
 CFS_INTERNAL int64_t batch_problem_tiles(const cfs_batch* b, const cfs_config* cfg);
 CFS_INTERNAL int64_t batch_resident_workgroups(const cfs_batch* b);
+// Device memory free on the batch's device now (hipMemGetInfo).
+CFS_INTERNAL int32_t batch_device_free_bytes(const cfs_batch* b, int64_t* bytes);
 // Makes the batch's device current on the calling thread (a worker thread
 // starts on the default device).
 CFS_INTERNAL int32_t batch_make_device_current(const cfs_batch* b);

@@ -393,11 +393,14 @@ typedef struct cfs_request {
  * workgroups at once) is solved instead by worker threads through the
  * single-solve entry points (cfs_create, cfs_advance; its own stream and
  * device memory), up to two such requests at a time in arrival order,
- * alongside the rounds; its result is the same as in the batch.  Returns when every request
- * has finished; a request the batch would refuse is CFS_E_ARG (or
+ * alongside the rounds; its result is the same as in the batch.  The solo
+ * lane admits by footprint as well: its handles may hold at most nine tenths
+ * of the device memory free when cfs_serve starts, and a worker keeps its
+ * handle for a next request of the same configuration.  Returns when every
+ * request has finished; a request the batch would refuse is CFS_E_ARG (or
  * CFS_E_UNSUPPORTED), and one whose footprint exceeds the whole arena without
- * filling the GPU is CFS_E_FULL, before anything runs.  No reference
- * counterpart. */
+ * filling the GPU, or the solo lane's budget when it does, is CFS_E_FULL,
+ * before anything runs.  No reference counterpart. */
 int32_t cfs_serve(cfs_batch* b, cfs_request* reqs, int32_t n, int32_t steps_per_poll);
 
 #ifdef __cplusplus

@@ -71,8 +71,13 @@ def test_serve_solo_lane_for_requests_that_fill_the_gpu(precision):
                                 cylinder_ny=(32, 48), t_end_riemann=(0.01, 0.02), t_end_cylinder=(0.05, 0.08))
     big_cfg, big_U = cfs.riemann_batch_problems(17, count=1, sizes=(1024,), precision=precision, rk=2)[0]
     reqs.insert(1, dict(cfg=big_cfg, U=big_U, ib=None, cfl=0.5, t_end=0.002, arrival_s=0.0))
+    # three more of the same configuration (other states and end times): a
+    # solo-lane worker keeps its solver handle for them
+    for k, (c2, U2) in enumerate(cfs.riemann_batch_problems(18, count=3, sizes=(1024,), precision=precision,
+                                                            rk=2)):
+        reqs.append(dict(cfg=big_cfg, U=U2, ib=None, cfl=0.5, t_end=0.001 * (k + 1), arrival_s=0.0))
     fps = [cfs.Batch.footprint(r["cfg"], r["ib"]["n_ig"] if r["ib"] is not None else 0) for r in reqs]
-    arena = 3 * max(fps[:1] + fps[2:])
+    arena = 3 * max(fps[:1] + fps[2:-3])
     assert fps[1] > arena
     with cfs.Batch(rk=2, precision=precision, max_problems=8, arena_bytes=arena, limiter=True) as b:
         res = b.serve(reqs, steps_per_poll=3)
