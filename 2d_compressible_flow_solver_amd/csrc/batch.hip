@@ -385,6 +385,9 @@ int32_t cfs_batch_admit(cfs_batch* b, const cfs_config* cfg, const double* U, co
             StageArgs& a = b->descs_host[q * 3 + k];
             if (k < cfg->rk) fill_stage_args(a, d, k, q, k & 1, cfl_stage_time(cfg->rk, k, cfl, t_end));
             else a = StageArgs{};
+            // the batch's resident state as a whole, not one problem's, decides
+            // whether the update streams past the last-level cache
+            a.stream_nt = b->bc.arena_bytes > LLC_BYTES ? 1 : 0;
         }
     if (hipMemcpyAsync(b->descs + (size_t)DESCS_PER_SLOT * p, b->descs_host, sizeof(StageArgs) * DESCS_PER_SLOT,
                        hipMemcpyHostToDevice, b->stream) != hipSuccess ||

@@ -87,6 +87,11 @@ struct ProblemDev {
     ScalPtrs s() const { return ScalPtrs{scal}; }
 };
 
+// Last-level cache of the device (MI355X Infinity Cache, 256 MiB): state
+// plane-sets beyond it are streamed with non-temporal accesses in the RK
+// update (StageArgs::stream_nt).
+constexpr long long LLC_BYTES = 256ll << 20;
+
 // How a stage obtains dt: fixed (dt scalar set by the host), or as part of a
 // CFL step (first stage derives dt from the bound of the entering state, the
 // last stage accumulates the bound of the leaving state).
@@ -117,6 +122,7 @@ inline int fill_stage_args(StageArgs& a, const ProblemDev& p, int sub, int step_
     a.inv_dx = 1.0 / a.dx;
     a.inv_dy = 1.0 / a.dy;
     int e;
+    a.stream_nt = 3ll * 32 * p.plane > LLC_BYTES ? 1 : 0;   // three plane-sets of 32 B per cell
     a.dx_pow2 = std::frexp(a.dx, &e) == 0.5;   // then 1/dx is exact and x/dx == x*(1/dx)
     a.dy_pow2 = std::frexp(a.dy, &e) == 0.5;
     a.gamma = c.gamma;

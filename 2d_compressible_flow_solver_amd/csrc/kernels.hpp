@@ -84,6 +84,9 @@ struct StageArgs {
     int positivity;       // inadmissible WENO face state -> adjacent cell state
     int hll_bounds;       // 0: reference (momentum, left state in SR), 1: velocity bounds
     int dx_pow2, dy_pow2; // dx (dy) is a power of two: divide by multiplying with 1/dx
+    int stream_nt;        // the RK snapshots are read and the new state written with non-temporal
+                          // accesses (state planes larger than the last-level cache: their lines
+                          // are not reused before eviction, and would push out staged halo lines)
     double dx, dy, inv_dx, inv_dy, gamma;
     double c0, c1, c2;
     int bc[4];
@@ -899,6 +902,39 @@ __device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, d
     }
 }
 
+// The TVD-RK combination of one cell (TVD2rdRK.cpp:3-31, TVD3rdRK.cpp:3-48)
+// from its state u, right-hand side L and the RK snapshots, and the store of
+// the new state (immersed ghosts were written while staging the tile).  NT:
+// the snapshots are read and the state written with non-temporal accesses.
+template <bool NT>
+__device__ __forceinline__ void update_cell(const StageArgs& a, long long c, int ty, double dt, const double u[4],
+                                            const double L[4], double res[4]) {
+    const double c2dt = a.c2 * dt;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const gptr<const double> p0 = G(a.u0) + q * a.plane + c, p1 = G(a.u1) + q * a.plane + c;
+        res[q] = u[q];
+        if (ty == 0) {
+            if (a.form == 0) res[q] = a.c1 * u[q] + c2dt * L[q];
+            else if (a.form == 1) res[q] = (a.c0 * (NT ? __builtin_nontemporal_load(p0) : *p0) + a.c1 * u[q]) + c2dt * L[q];
+            else if (a.form == 2)
+                res[q] = (a.c0 * (NT ? __builtin_nontemporal_load(p0) : *p0) +
+                          a.c1 * (NT ? __builtin_nontemporal_load(p1) : *p1)) + c2dt * L[q];
+            else res[q] = L[q];
+        } else if (a.form == 3) {
+            res[q] = 0.0;
+        }
+    }
+    if (ty != 2 || a.form == 3) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const gptr<double> o = G(a.out) + q * a.plane + c;
+            if (NT) __builtin_nontemporal_store(res[q], o);
+            else *o = res[q];
+        }
+    }
+}
+
 template <typename R, bool VISC, bool LIMIT, typename Sh, bool SPLIT = true, bool HALF = false>
 __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R* F, double* wmax,
                                            ViscLDS<R, Sh>* V, LimitLDS<R, Sh>* Lm) {
@@ -1075,34 +1111,30 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
             const long long c = (long long)i * a.ny + j;
             const int ty = (a.type == nullptr) ? 0 : G(a.type)[c];
             const R rdy = (R)a.dy, idy = (R)a.inv_dy;
-            const double c2dt = a.c2 * dt;
-            double res[4];
+            double u[4], L[4], res[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 // the FP64 tile holds the exact state; FP32-bulk re-reads it
-                const double u = (sizeof(R) == 8) ? (double)S[q * NS + (il + HALO) * SC + jl + HALO]
-                                                  : G(a.in)[q * a.plane + c];
-                res[q] = u;
+                u[q] = (sizeof(R) == 8) ? (double)S[q * NS + (il + HALO) * SC + jl + HALO] : G(a.in)[q * a.plane + c];
+                L[q] = 0.0;
                 if (ty == 0) {
                     const R gb = F[q * NF + YF + il * (TJ + 1) + jl];
                     const R gt = F[q * NF + YF + il * (TJ + 1) + jl + 1];
                     const R txq = HALF ? tx[q] : rhs_x<R, Sh, HALF>(a, F, q, il, jl);
                     // x/d == x*(1/d) bit for bit when d is a power of two
                     const R tyy = a.dy_pow2 ? (gt - gb) * idy : K<R>::div(gt - gb, rdy);
-                    const R rhs = txq - tyy;
-                    const double L = (double)rhs;
-                    if (a.form == 0) res[q] = a.c1 * u + c2dt * L;
-                    else if (a.form == 1) res[q] = (a.c0 * G(a.u0)[q * a.plane + c] + a.c1 * u) + c2dt * L;
-                    else if (a.form == 2) res[q] = (a.c0 * G(a.u0)[q * a.plane + c] + a.c1 * G(a.u1)[q * a.plane + c]) + c2dt * L;
-                    else res[q] = L;
-                } else if (a.form == 3) {
-                    res[q] = 0.0;
+                    L[q] = (double)(txq - tyy);
                 }
             }
-            // immersed ghosts were written while staging the tile
-            if (ty != 2 || a.form == 3) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) G(a.out)[q * a.plane + c] = res[q];
+            // the two forms of the snapshot reads and state writes stay apart
+            // (the empty asm statements keep the compiler from merging them,
+            // which would drop the non-temporal hint)
+            if (a.stream_nt) {
+                asm volatile("" ::: "memory");
+                update_cell<true>(a, c, ty, dt, u, L, res);
+                asm volatile("" ::: "memory");
+            } else {
+                update_cell<false>(a, c, ty, dt, u, L, res);
             }
             if (a.fuse_cfl && ty == 0)
                 smax = (sizeof(R) == 8)
