@@ -326,7 +326,7 @@ def device_count():
     return _lib().cfs_device_count()
 
 
-NUM_TILE_SHAPES = 3
+NUM_TILE_SHAPES = 4
 
 
 def tile_shape_info(shape):

@@ -114,6 +114,7 @@ int launch_batch_stage(cfs_batch* b, int parity, int sub) {
     const int k = b->bc.tile_shape;
     return k == 1 ? launch_batch_shape<R, LIMIT, Shape1>(b, parity, sub)
                   : k == 2 ? launch_batch_shape<R, LIMIT, Shape2>(b, parity, sub)
+                  : k == 3 ? launch_batch_shape<R, LIMIT, Shape3>(b, parity, sub)
                            : launch_batch_shape<R, LIMIT, Shape0>(b, parity, sub);
 }
 
@@ -125,7 +126,8 @@ hipError_t blocks_per_cu(int* per_cu) {
 template <typename R, bool LIMIT>
 hipError_t blocks_per_cu_shape(int k, int* per_cu) {
     return k == 1 ? blocks_per_cu<R, LIMIT, Shape1>(per_cu)
-                  : k == 2 ? blocks_per_cu<R, LIMIT, Shape2>(per_cu) : blocks_per_cu<R, LIMIT, Shape0>(per_cu);
+                  : k == 2 ? blocks_per_cu<R, LIMIT, Shape2>(per_cu)
+                  : k == 3 ? blocks_per_cu<R, LIMIT, Shape3>(per_cu) : blocks_per_cu<R, LIMIT, Shape0>(per_cu);
 }
 
 int launch_batch_step(cfs_batch* b, int parity) {

@@ -89,7 +89,8 @@ template <typename R, bool VISC, bool LIMIT>
 void launch_shape(int shape, dim3 grid, hipStream_t stream, const StageArgs& a) {
     if (VISC || shape == 0) launch_stage_kernel<R, VISC, LIMIT, Shape0>(grid, stream, a);
     else if (shape == 1) launch_stage_kernel<R, false, LIMIT, Shape1>(grid, stream, a);
-    else launch_stage_kernel<R, false, LIMIT, Shape2>(grid, stream, a);
+    else if (shape == 2) launch_stage_kernel<R, false, LIMIT, Shape2>(grid, stream, a);
+    else launch_stage_kernel<R, false, LIMIT, Shape3>(grid, stream, a);
 }
 
 // the kernel variant of a configuration: precision, viscous terms, limiter,
@@ -108,6 +109,13 @@ void dispatch_stage(const cfs_config& c, int shape, dim3 grid, hipStream_t strea
 }
 
 dim3 stage_grid(const cfs_solver* s) { return dim3((unsigned)(8 * ((s->dev.tiles_i * s->dev.tiles_j + 7) / 8))); }
+
+// The tile shape a new handle starts with: the dual-face shape 3 for
+// inviscid grids of at most 5000 tiles of 15 x 16 (up to about 1100^2
+// cells), where its two chains per thread beat shape 0 by 1-4% (Riemann
+// problems 256^2 to 1024^2, Lax-Liu 400^2, both precisions); shape 0 above
+// (the 2048 x 1024 cylinder: 1% the other way).  cfs_autotune measures instead.
+int default_shape(const cfs_config& c) { return (!c.viscous && tiles_for(c, 0) <= 5000) ? 3 : 0; }
 
 // Makes tile shape k current: tile counts and the IB tile flags of that shape.
 void apply_shape(cfs_solver* s, int k) {
@@ -227,7 +235,7 @@ int32_t cfs_create(const cfs_config* cfg, void* hip_stream, cfs_solver** out) {
     ProblemDev& d = s->dev;
     d.cfg = &s->cfg;
     d.plane = (long long)cfg->nx * cfg->ny;
-    apply_shape(s, 0);
+    apply_shape(s, default_shape(*cfg));
     if (hip_stream) {
         s->stream = (hipStream_t)hip_stream;
     } else {

@@ -189,6 +189,7 @@ struct ShapeDims {
 inline ShapeDims shape_dims(int k) {
     if (k == 1) return ShapeDims{Shape1::TI, Shape1::TJ, Shape1::NT};
     if (k == 2) return ShapeDims{Shape2::TI, Shape2::TJ, Shape2::NT};
+    if (k == 3) return ShapeDims{Shape3::TI, Shape3::TJ, Shape3::NT};
     return ShapeDims{Shape0::TI, Shape0::TJ, Shape0::NT};
 }
 inline long long tiles_for(const cfs_config& c, int k) {

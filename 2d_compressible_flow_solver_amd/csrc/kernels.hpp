@@ -52,8 +52,13 @@ template <typename T> __device__ __forceinline__ gptr<T> G(T* p) { return (gptr<
 // i, TJ columns along y / j, j contiguous in memory) and stages SR x SC cells
 // (the tile plus the halo) in LDS.  Its (TI + 1) TJ x-faces and TI (TJ + 1)
 // y-faces take two rounds of NT threads; every cell update one thread.
-template <int TI_, int TJ_, int NT_> struct TileShape {
+template <int TI_, int TJ_, int NT_, bool DUAL_ = false> struct TileShape {
     static constexpr int TI = TI_, TJ = TJ_, NT = NT_;
+    // DUAL: each thread computes its x-face and its y-face in one pass (two
+    // independent dependency chains per thread: for grids too small to give
+    // every SIMD enough waves, where the per-face latency, not the issue
+    // rate, bounds the stage)
+    static constexpr bool DUAL = DUAL_;
     static constexpr int SR = TI + 2 * HALO, SC = TJ + 2 * HALO;
     static constexpr int NXF = (TI + 1) * TJ, NYF = TI * (TJ + 1);
     static_assert(NXF + NYF <= 2 * NT && TI * TJ <= NT && NT % 64 == 0, "two face rounds, one cell per thread");
@@ -65,7 +70,8 @@ template <int TI_, int TJ_, int NT_> struct TileShape {
 using Shape0 = TileShape<15, 16, 256>;    // 240 cells, 511 faces (256 + 255)
 using Shape1 = TileShape<15, 32, 512>;    // 480 cells, 1007 faces (512 + 495)
 using Shape2 = TileShape<31, 32, 1024>;   // 992 cells, 2047 faces (1024 + 1023)
-constexpr int NUM_SHAPES = 3;
+using Shape3 = TileShape<15, 16, 256, true>;   // as Shape0, x- and y-face per thread in one pass
+constexpr int NUM_SHAPES = 4;
 
 enum { BC_NEUMANN = 0, BC_DIRICHLET = 1, BC_PERIODIC = 2 };
 
@@ -330,61 +336,83 @@ __device__ __forceinline__ void face_flux(const R* __restrict__ S, int ns, const
     riemann_flux<R>(L, Rr, cl, cr, g, positivity, hll_bounds, out);
 }
 
-// Pressures, the positivity fallback, Euler fluxes and the HLL flux of a
-// face's two reconstructed states (cl, cr: the adjacent cells' states).
+// Pressures, the positivity fallback, Euler fluxes and the HLL flux of N
+// faces' two reconstructed states each (cl, cr: the adjacent cells' states).
+// The faces are independent; each statement runs over all N before the next
+// one, so N > 1 gives the scheduler N independent dependency chains.
+template <typename R, int N>
+__device__ __forceinline__ void riemann_flux_n(const R (*L)[4], const R (*Rr)[4], const R (*cl)[4], const R (*cr)[4],
+                                               R g, int positivity, int hll_bounds, R (*out)[4]) {
+    using C = K<R>;
+    const R gm1 = g - R(1);
+    R rL[N], nL[N], tL[N], EL[N], rR[N], nR[N], tR[N], ER[N];
+    R pL[N], unL[N], nnL[N], ntL[N], gpL[N], pR[N], unR[N], nnR[N], ntR[N], gpR[N];
+#pragma unroll
+    for (int f = 0; f < N; ++f) {
+        rL[f] = L[f][0]; nL[f] = L[f][1]; tL[f] = L[f][2]; EL[f] = L[f][3];
+        rR[f] = Rr[f][0]; nR[f] = Rr[f][1]; tR[f] = Rr[f][2]; ER[f] = Rr[f][3];
+        // pressure and the state's divisions by its density (EulerPhase.cpp:132,
+        // EulerMod.cpp:35-50, RiemmanSolverHLL.cpp:18-23)
+        state_terms<R>(rL[f], nL[f], tL[f], EL[f], g, gm1, pL[f], unL[f], nnL[f], ntL[f], gpL[f]);
+        state_terms<R>(rR[f], nR[f], tR[f], ER[f], g, gm1, pR[f], unR[f], nnR[f], ntR[f], gpR[f]);
+    }
+    if (positivity) {
+#pragma unroll
+        for (int f = 0; f < N; ++f) {
+            // an inadmissible reconstruction takes the state of the cell on its side
+            if (!(rL[f] > R(0) && pL[f] > R(0))) {
+                rL[f] = cl[f][0]; nL[f] = cl[f][1]; tL[f] = cl[f][2]; EL[f] = cl[f][3];
+                state_terms<R>(rL[f], nL[f], tL[f], EL[f], g, gm1, pL[f], unL[f], nnL[f], ntL[f], gpL[f]);
+            }
+            if (!(rR[f] > R(0) && pR[f] > R(0))) {
+                rR[f] = cr[f][0]; nR[f] = cr[f][1]; tR[f] = cr[f][2]; ER[f] = cr[f][3];
+                state_terms<R>(rR[f], nR[f], tR[f], ER[f], g, gm1, pR[f], unR[f], nnR[f], ntR[f], gpR[f]);
+            }
+        }
+    }
+#pragma unroll
+    for (int f = 0; f < N; ++f) {
+        // Euler fluxes of both states
+        const R FL0 = nL[f], FL1 = nnL[f] + pL[f], FL2 = ntL[f], FL3 = (EL[f] + pL[f]) * unL[f];
+        const R FR0 = nR[f], FR1 = nnR[f] + pR[f], FR2 = ntR[f], FR3 = (ER[f] + pR[f]) * unR[f];
+        // HLL with PVRS star-state speed estimates
+        const R aL = C::sqrt_(gpL[f]);
+        const R aR = C::sqrt_(gpR[f]);
+        const R a_ave = C::half * (aL + aR);
+        const R rho_ave = C::half * (rL[f] + rR[f]);
+        const R p_star = C::madd(-((C::half * (unR[f] - unL[f])) * rho_ave), a_ave, C::half * (pL[f] + pR[f]));
+        const R u_star = C::half * (unR[f] + unL[f]) - C::div(C::half * (pR[f] - pL[f]), rho_ave * a_ave);
+        const R rsL = rL[f] + C::div((unL[f] - u_star) * rho_ave, a_ave);
+        const R rsR = rR[f] - C::div((unR[f] - u_star) * rho_ave, a_ave);
+        const R asL = C::sqrt_(C::div(g * p_star, rsL));
+        const R asR = C::sqrt_(C::div(g * p_star, rsR));
+        // reference bounds: normal momentum, left state in SR; else velocities
+        const R x1 = (hll_bounds ? unL[f] : nL[f]) - aL, x2 = u_star - asL;
+        const R SL = (x2 < x1) ? x2 : x1;           // std::min
+        const R y1 = hll_bounds ? unR[f] + aR : nL[f] + aL, y2 = u_star + asR;
+        const R SR = (y1 < y2) ? y2 : y1;           // std::max
+        const R inv = C::div(R(1), SR - SL);
+        const R ss = SL * SR;
+        const R H0 = C::madd(ss, rR[f] - rL[f], C::madd(SR, FL0, -(SL * FR0))) * inv;
+        const R H1 = C::madd(ss, nR[f] - nL[f], C::madd(SR, FL1, -(SL * FR1))) * inv;
+        const R H2 = C::madd(ss, tR[f] - tL[f], C::madd(SR, FL2, -(SL * FR2))) * inv;
+        const R H3 = C::madd(ss, ER[f] - EL[f], C::madd(SR, FL3, -(SL * FR3))) * inv;
+        const bool left = SL >= R(0), right = SR <= R(0);
+        out[f][0] = left ? FL0 : right ? FR0 : H0;
+        out[f][1] = left ? FL1 : right ? FR1 : H1;
+        out[f][2] = left ? FL2 : right ? FR2 : H2;
+        out[f][3] = left ? FL3 : right ? FR3 : H3;
+    }
+}
+
 template <typename R>
 __device__ __forceinline__ void riemann_flux(const R L[4], const R Rr[4], const R cl[4], const R cr[4], R g,
                                              int positivity, int hll_bounds, R out[4]) {
-    using C = K<R>;
-    const R gm1 = g - R(1);
-    R rL = L[0], nL = L[1], tL = L[2], EL = L[3];
-    R rR = Rr[0], nR = Rr[1], tR = Rr[2], ER = Rr[3];
-    // pressure and the state's divisions by its density (EulerPhase.cpp:132,
-    // EulerMod.cpp:35-50, RiemmanSolverHLL.cpp:18-23)
-    R pL, unL, nnL, ntL, gpL, pR, unR, nnR, ntR, gpR;
-    state_terms<R>(rL, nL, tL, EL, g, gm1, pL, unL, nnL, ntL, gpL);
-    state_terms<R>(rR, nR, tR, ER, g, gm1, pR, unR, nnR, ntR, gpR);
-    if (positivity) {
-        // an inadmissible reconstruction takes the state of the cell on its side
-        if (!(rL > R(0) && pL > R(0))) {
-            rL = cl[0]; nL = cl[1]; tL = cl[2]; EL = cl[3];
-            state_terms<R>(rL, nL, tL, EL, g, gm1, pL, unL, nnL, ntL, gpL);
-        }
-        if (!(rR > R(0) && pR > R(0))) {
-            rR = cr[0]; nR = cr[1]; tR = cr[2]; ER = cr[3];
-            state_terms<R>(rR, nR, tR, ER, g, gm1, pR, unR, nnR, ntR, gpR);
-        }
-    }
-    // Euler fluxes of both states
-    const R FL0 = nL, FL1 = nnL + pL, FL2 = ntL, FL3 = (EL + pL) * unL;
-    const R FR0 = nR, FR1 = nnR + pR, FR2 = ntR, FR3 = (ER + pR) * unR;
-    // HLL with PVRS star-state speed estimates
-    const R aL = C::sqrt_(gpL);
-    const R aR = C::sqrt_(gpR);
-    const R a_ave = C::half * (aL + aR);
-    const R rho_ave = C::half * (rL + rR);
-    const R p_star = C::madd(-((C::half * (unR - unL)) * rho_ave), a_ave, C::half * (pL + pR));
-    const R u_star = C::half * (unR + unL) - C::div(C::half * (pR - pL), rho_ave * a_ave);
-    const R rsL = rL + C::div((unL - u_star) * rho_ave, a_ave);
-    const R rsR = rR - C::div((unR - u_star) * rho_ave, a_ave);
-    const R asL = C::sqrt_(C::div(g * p_star, rsL));
-    const R asR = C::sqrt_(C::div(g * p_star, rsR));
-    // reference bounds: normal momentum, left state in SR; else velocities
-    const R x1 = (hll_bounds ? unL : nL) - aL, x2 = u_star - asL;
-    const R SL = (x2 < x1) ? x2 : x1;           // std::min
-    const R y1 = hll_bounds ? unR + aR : nL + aL, y2 = u_star + asR;
-    const R SR = (y1 < y2) ? y2 : y1;           // std::max
-    const R inv = C::div(R(1), SR - SL);
-    const R ss = SL * SR;
-    const R H0 = C::madd(ss, rR - rL, C::madd(SR, FL0, -(SL * FR0))) * inv;
-    const R H1 = C::madd(ss, nR - nL, C::madd(SR, FL1, -(SL * FR1))) * inv;
-    const R H2 = C::madd(ss, tR - tL, C::madd(SR, FL2, -(SL * FR2))) * inv;
-    const R H3 = C::madd(ss, ER - EL, C::madd(SR, FL3, -(SL * FR3))) * inv;
-    const bool left = SL >= R(0), right = SR <= R(0);
-    out[0] = left ? FL0 : right ? FR0 : H0;
-    out[1] = left ? FL1 : right ? FR1 : H1;
-    out[2] = left ? FL2 : right ? FR2 : H2;
-    out[3] = left ? FL3 : right ? FR3 : H3;
+    R o[1][4];
+    riemann_flux_n<R, 1>((const R(*)[4])L, (const R(*)[4])Rr, (const R(*)[4])cl, (const R(*)[4])cr, g, positivity,
+                         hll_bounds, o);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) out[q] = o[0][q];
 }
 
 // Fetch one cell of the (halo-extended) tile, applying the physical boundary:
@@ -840,6 +868,80 @@ __device__ __forceinline__ void tile_face(const StageArgs& a, int s, int i0, int
     F[3 * NF + e] = fl[3];
 }
 
+// x-face s and y-face s of a tile (the dual-face tile shape), computed
+// together: the WENO5 reconstructions of both faces and then their HLL fluxes
+// run statement by statement over the two faces, two independent dependency
+// chains per thread.  Stencils, arithmetic and flux slots are tile_face's.
+// The tile has one y-face fewer than x-faces: the last thread computes a copy
+// of the y-face before it and does not store it.
+template <typename R, bool LIMIT, bool FAST, typename Sh>
+__device__ __forceinline__ void tile_face_pair(const StageArgs& a, int s, int i0, int j0, R g, R lam_x, R lam_y,
+                                               const R* S, R* F, const LimitLDS<R, Sh>* Lm) {
+    constexpr int TJ = Sh::TJ, SC = Sh::SC, NXF = Sh::NXF, NYF = Sh::NYF;
+    constexpr int NS = Sh::SR * Sh::SC, NF = NXF + NYF;
+    const int sy = s < NYF ? s : NYF - 1;
+    int cell[2][6];
+    {
+        const bool px = a.bc[0] == BC_PERIODIC;
+        const int fr = s / TJ, jl = s - fr * TJ;
+        const int f = i0 + fr;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            int o = k - 3;
+            if (!px && f == 0 && o > 0) o = 0;
+            if (!px && f == a.nx && o < -1) o = -1;
+            cell[0][k] = (fr + o + HALO) * SC + jl + HALO;
+        }
+    }
+    {
+        const bool py = a.bc[2] == BC_PERIODIC;
+        const int il = sy / (TJ + 1), gl = sy - il * (TJ + 1);
+        const int gf = j0 + gl;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            int o = k - 3;
+            if (!py && gf == 0 && o > 0) o = 0;
+            if (!py && gf == a.ny && o < -1) o = -1;
+            cell[1][k] = (il + HALO) * SC + gl + o + HALO;
+        }
+    }
+    // variables in each face's normal frame: (rho, m_n, m_t, E)
+    constexpr int comp[2][4] = {{0, 1, 2, 3}, {0, 2, 1, 3}};
+    R L[2][4], Rr[2][4], cl[2][4], cr[2][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+#pragma unroll
+        for (int f = 0; f < 2; ++f) {
+            R v[6];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) v[k] = S[comp[f][q] * NS + cell[f][k]];
+            L[f][q] = weno5<R, FAST>(v[0], v[1], v[2], v[3], v[4]);
+            Rr[f][q] = weno5<R, FAST>(v[5], v[4], v[3], v[2], v[1]);
+            cl[f][q] = v[2];
+            cr[f][q] = v[3];
+        }
+    }
+    R fl[2][4];
+    riemann_flux_n<R, 2>(L, Rr, cl, cr, g, a.positivity, a.hll_bounds, fl);
+    if constexpr (LIMIT) {
+        if (a.positivity == 2) {
+            limit_face<R, Sh>(S, NS, *Lm, cell[0][2], cell[0][3], 1, 2, SC, 1, lam_x, lam_y, g, fl[0]);
+            limit_face<R, Sh>(S, NS, *Lm, cell[1][2], cell[1][3], 2, 1, 1, SC, lam_y, lam_x, g, fl[1]);
+        }
+    }
+    F[s] = fl[0][0];
+    F[NF + s] = fl[0][1];
+    F[2 * NF + s] = fl[0][2];
+    F[3 * NF + s] = fl[0][3];
+    if (s < NYF) {
+        const int e = NXF + s;
+        F[e] = fl[1][0];
+        F[2 * NF + e] = fl[1][1];
+        F[NF + e] = fl[1][2];
+        F[3 * NF + e] = fl[1][3];
+    }
+}
+
 // x-face part of a cell's flux divergence, -(F(i+1/2) - F(i-1/2)) / dx, the
 // first operand of Cell::comRHS's difference (Cell.cpp:110-125)
 template <typename R, typename Sh, bool HALF>
@@ -868,7 +970,10 @@ __device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, d
     // bit for bit when d is a power of two)
     const R lam_x = (R)(a.dx_pow2 ? dt * a.inv_dx : dt / a.dx);
     const R lam_y = (R)(a.dy_pow2 ? dt * a.inv_dy : dt / a.dy);
-    if constexpr (SPLIT) {
+    if constexpr (SPLIT && Sh::DUAL) {
+        static_assert(!HALF && !VISC, "both directions' fluxes are live at once; inviscid only");
+        for (int s = tid; s < NXF; s += Sh::NT) tile_face_pair<R, LIMIT, FAST, Sh>(a, s, i0, j0, g, lam_x, lam_y, S, F, Lm);
+    } else if constexpr (SPLIT) {
         // a direction whose tile faces include no degenerate boundary face
         // (the fast division forms only: the IEEE form serves rare tiles)
         const bool xreg = FAST && (a.bc[0] == BC_PERIODIC || (i0 > 0 && i0 + Sh::TI < a.nx));
@@ -1155,8 +1260,10 @@ template <typename R, bool VISC, bool LIMIT, typename Sh> struct Occupancy {
     // the single-solve FP64 limiter kernel keeps one flux plane-set of x-faces
     // (HALF): 30 KB instead of 38 KB of LDS per 15 x 16 tile, five workgroups
     // per CU instead of four
-    static constexpr bool half_flux = sizeof(R) == 8 && LIMIT && !VISC;
-    static constexpr int want = VISC ? (sizeof(R) == 8 ? 3 : 6) : LIMIT ? (sizeof(R) == 8 ? 5 : 6) : (sizeof(R) == 8 ? 5 : 6);
+    static constexpr bool half_flux = sizeof(R) == 8 && LIMIT && !VISC && !Sh::DUAL;
+    static constexpr int want = VISC ? (sizeof(R) == 8 ? 3 : 6)
+                              : Sh::DUAL ? 4
+                              : LIMIT ? (sizeof(R) == 8 ? 5 : 6) : (sizeof(R) == 8 ? 5 : 6);
     static constexpr int extra = VISC ? (int)sizeof(ViscLDS<R, Sh>) : LIMIT ? (int)sizeof(LimitLDS<R, Sh>) : 0;
     static constexpr int lds = (int)sizeof(R) * (Sh::SR * Sh::SC * 4 + Sh::template flux_plane<half_flux>() * 4) +
                                8 * (Sh::NT / 64) + extra;
