@@ -571,6 +571,8 @@ void viscous_flux(const Oracle& o, const int idx[6], int line, bool xdir, double
 // face-normal frame (n: normal momentum, t: tangential): F = F_LF +
 // theta (F_hat - F_LF), theta the largest value in [0, 1] keeping both
 // adjacent cells' shares U -+ k (F - F(U)) admissible, k = 2 (ln an + lt at)/an.
+// theta = 1 at once when both shares taken with F_hat keep rho >= 1e-13 and
+// rho p >= 1e-13 rho (a floor every limited share also respects).
 
 double maxr(double x, double y) { return (x < y) ? y : x; }
 
@@ -638,12 +640,6 @@ void limit_face(const Oracle& o, const Phase& cL, const Phase& cR, int iL, int j
     double FL[4], FR[4];
     euler_flux_n(g, UL, FL);
     euler_flux_n(g, UR, FR);
-    const double a = maxr(wave_speed(o, iL, jL, xdir), wave_speed(o, iR, jR, xdir));
-    double Flf[4], D[4];
-    for (int q = 0; q < 4; ++q) {
-        Flf[q] = 0.5 * (FL[q] + FR[q]) - (0.5 * a) * (UR[q] - UL[q]);
-        D[q] = fh[q] - Flf[q];
-    }
     const int di = xdir ? 1 : 0, dj = xdir ? 0 : 1;   // neighbour offsets along the normal
     auto share_k = [&](int i, int j) {
         const double an = maxr(maxr(wave_speed(o, i - di, j - dj, xdir), wave_speed(o, i, j, xdir)),
@@ -654,17 +650,39 @@ void limit_face(const Oracle& o, const Phase& cL, const Phase& cR, int iL, int j
         return 2.0 * (ln * an + lt * at) / an;
     };
     const double kL = share_k(iL, jL), kR = share_k(iR, jR);
+    // both shares with the face's own flux above the fixed floor (rho >= eps,
+    // rho p >= eps rho, eps = 1e-13): theta = 1, the flux stays
+    const double eps = 1e-13;
+    double HL[4], HR[4];
+    for (int q = 0; q < 4; ++q) {
+        HL[q] = UL[q] - kL * (fh[q] - FL[q]);
+        HR[q] = UR[q] + kR * (fh[q] - FR[q]);
+    }
+    const bool accL = HL[0] >= eps && rho_p(g, HL[0], HL[1], HL[2], HL[3]) >= eps * HL[0];
+    const bool accR = HR[0] >= eps && rho_p(g, HR[0], HR[1], HR[2], HR[3]) >= eps * HR[0];
+    if (accL && accR) return;
+    const double a = maxr(wave_speed(o, iL, jL, xdir), wave_speed(o, iR, jR, xdir));
+    double Flf[4], D[4];
+    for (int q = 0; q < 4; ++q) {
+        Flf[q] = 0.5 * (FL[q] + FR[q]) - (0.5 * a) * (UR[q] - UL[q]);
+        D[q] = fh[q] - Flf[q];
+    }
     double B0[4], Dl[4];
-    for (int q = 0; q < 4; ++q) {
-        B0[q] = UL[q] - kL * (Flf[q] - FL[q]);
-        Dl[q] = -(kL * D[q]);
+    double thL = 1.0, thR = 1.0;
+    if (!accL) {
+        for (int q = 0; q < 4; ++q) {
+            B0[q] = UL[q] - kL * (Flf[q] - FL[q]);
+            Dl[q] = -(kL * D[q]);
+        }
+        thL = share_theta(g, B0, Dl);
     }
-    const double thL = share_theta(g, B0, Dl);
-    for (int q = 0; q < 4; ++q) {
-        B0[q] = UR[q] + kR * (Flf[q] - FR[q]);
-        Dl[q] = kR * D[q];
+    if (!accR) {
+        for (int q = 0; q < 4; ++q) {
+            B0[q] = UR[q] + kR * (Flf[q] - FR[q]);
+            Dl[q] = kR * D[q];
+        }
+        thR = share_theta(g, B0, Dl);
     }
-    const double thR = share_theta(g, B0, Dl);
     double th = (thR < thL) ? thR : thL;
     th = (th < 0.0) ? 0.0 : th;
     if (th < 1.0) {
