@@ -391,17 +391,22 @@ __device__ __forceinline__ void riemann_flux_n(const R (*L)[4], const R (*Rr)[4]
         const R SL = (x2 < x1) ? x2 : x1;           // std::min
         const R y1 = hll_bounds ? unR[f] + aR : nL[f] + aL, y2 = u_star + asR;
         const R SR = (y1 < y2) ? y2 : y1;           // std::max
-        const R inv = C::div(R(1), SR - SL);
-        const R ss = SL * SR;
-        const R H0 = C::madd(ss, rR[f] - rL[f], C::madd(SR, FL0, -(SL * FR0))) * inv;
-        const R H1 = C::madd(ss, nR[f] - nL[f], C::madd(SR, FL1, -(SL * FR1))) * inv;
-        const R H2 = C::madd(ss, tR[f] - tL[f], C::madd(SR, FL2, -(SL * FR2))) * inv;
-        const R H3 = C::madd(ss, ER[f] - EL[f], C::madd(SR, FL3, -(SL * FR3))) * inv;
         const bool left = SL >= R(0), right = SR <= R(0);
-        out[f][0] = left ? FL0 : right ? FR0 : H0;
-        out[f][1] = left ? FL1 : right ? FR1 : H1;
-        out[f][2] = left ? FL2 : right ? FR2 : H2;
-        out[f][3] = left ? FL3 : right ? FR3 : H3;
+        if (left || right) {
+            // supersonic: one side's Euler flux, the HLL average is not formed
+            // (whole waves of a supersonic stream skip it)
+            out[f][0] = left ? FL0 : FR0;
+            out[f][1] = left ? FL1 : FR1;
+            out[f][2] = left ? FL2 : FR2;
+            out[f][3] = left ? FL3 : FR3;
+        } else {
+            const R inv = C::div(R(1), SR - SL);
+            const R ss = SL * SR;
+            out[f][0] = C::madd(ss, rR[f] - rL[f], C::madd(SR, FL0, -(SL * FR0))) * inv;
+            out[f][1] = C::madd(ss, nR[f] - nL[f], C::madd(SR, FL1, -(SL * FR1))) * inv;
+            out[f][2] = C::madd(ss, tR[f] - tL[f], C::madd(SR, FL2, -(SL * FR2))) * inv;
+            out[f][3] = C::madd(ss, ER[f] - EL[f], C::madd(SR, FL3, -(SL * FR3))) * inv;
+        }
     }
 }
 
@@ -606,10 +611,30 @@ __device__ __forceinline__ void block_max_to(double m, unsigned long long* acc, 
 // Lax-Friedrichs flux being formed: about 99.4% of the faces of a Mach-3
 // cylinder flow, whose other faces run the full test per wave.
 
-template <typename R, typename Sh> struct LimitLDS {
+// The limiter's per-cell values in LDS.  FP64 (CellK, the LDS bounds the
+// workgroups per CU): for the cells next to a tile face (rows -1..TI, columns
+// -1..TJ of the tile, corners unused) 1/rho and the pressure, and the share
+// factors k of the cells of x-faces (rows -1..TI, columns 0..TJ-1) and of
+// y-faces (rows 0..TI-1, columns -1..TJ), each computed once for both faces
+// of its cell; while the tile is staged, |u| + c and |v| + c of rows and
+// columns -2..TI+1 / -2..TJ+1 live in the flux planes, not yet written.
+// FP32 (FaceK): |u| + c, |v| + c, 1/rho and pressure of every staged cell in
+// the state planes' indexing, k per face (fewer integer offsets and no
+// second pass: the faster form where the LDS has room).
+template <typename R, typename Sh> struct LimitCellK {
+    static constexpr int PW = Sh::TJ + 2, NP = (Sh::TI + 2) * PW;     // 1/rho, p: row width, cells
+    static constexpr int NKX = (Sh::TI + 2) * Sh::TJ, NKY = Sh::TI * PW;
+    static constexpr int WW = Sh::TJ + 4, NW = (Sh::TI + 4) * WW;     // |u|+c, |v|+c: row width, cells
+    static_assert(2 * NW <= 4 * Sh::NYF, "the wave speeds fit in any flux plane-set");
+    R ir[NP], p[NP];
+    R kx[NKX], ky[NKY];
+};
+template <typename R, typename Sh> struct LimitFaceK {
     R sx[Sh::SR * Sh::SC], sy[Sh::SR * Sh::SC];   // |u| + c and |v| + c of every staged cell
     R ir[Sh::SR * Sh::SC], p[Sh::SR * Sh::SC];    // 1/rho and the pressure of every staged cell
 };
+template <typename R, typename Sh>
+using LimitLDS = typename std::conditional<sizeof(R) == 8, LimitCellK<R, Sh>, LimitFaceK<R, Sh>>::type;
 
 template <typename R> __device__ __forceinline__ R maxr(R x, R y) { return (x < y) ? y : x; }
 
@@ -682,26 +707,118 @@ __device__ __forceinline__ void euler_flux_n(R n, R t, R E, R ir, R p, R F[4]) {
     F[3] = (E + p) * un;
 }
 
-// cL, cR: LDS cells left and right of the face; dn, dt: LDS offsets of the
-// neighbours along the normal and the tangent; ln, lt: dt/h along them
+// |u| + c and |v| + c of the cells around a tile (the |u_n| + c of the
+// limiter) and 1/rho and pressure of the cells next to its faces; FP64 also
+// the share factors k = 2 (ln an + lt at) / an of those cells (an, at: the
+// largest |u_n| + c, |u_t| + c over the cell and its two neighbours along n,
+// t; ln, lt: dt/h along them).  W: scratch room for the FP64 wave speeds.
 template <typename R, typename Sh>
-__device__ __forceinline__ void limit_face(const R* __restrict__ S, int ns, const LimitLDS<R, Sh>& Lm, int cL, int cR, int nrm,
-                                           int tng, int dn, int dtg, R ln, R lt, R g, R fl[4]) {
+__device__ __forceinline__ void limit_stage(const R* __restrict__ S, R* __restrict__ W, LimitLDS<R, Sh>* Lm, R g,
+                                            R lam_x, R lam_y) {
     using C = K<R>;
-    const R* sn = (nrm == 1) ? Lm.sx : Lm.sy;
-    const R* st = (nrm == 1) ? Lm.sy : Lm.sx;
+    constexpr int TI = Sh::TI, TJ = Sh::TJ, SC = Sh::SC, NS = Sh::SR * Sh::SC;
+    const R gm1 = g - R(1);
+    if constexpr (sizeof(R) == 8) {
+        using LL = LimitCellK<R, Sh>;
+        R* sx = W;
+        R* sy = W + LL::NW;
+        for (int s = threadIdx.x; s < LL::NW; s += Sh::NT) {
+            const int r = s / LL::WW, c = s - r * LL::WW;   // tile row r - 2, column c - 2
+            const int e = (r + HALO - 2) * SC + c + HALO - 2;
+            const R rho = S[e], mx = S[NS + e], my = S[2 * NS + e], E = S[3 * NS + e];
+            const R ir = C::div(R(1), rho);
+            const R ux = mx * ir, uy = my * ir;
+            const R p = gm1 * (E - C::half * (mx * ux + my * uy));
+            const R cs = C::sqrt_((g * p) * ir);
+            sx[s] = (ux < R(0) ? -ux : ux) + cs;
+            sy[s] = (uy < R(0) ? -uy : uy) + cs;
+            if (r >= 1 && r <= TI + 2 && c >= 1 && c <= TJ + 2) {
+                const int q = (r - 1) * LL::PW + c - 1;
+                Lm->ir[q] = ir;
+                Lm->p[q] = p;
+            }
+        }
+        __syncthreads();
+        for (int s = threadIdx.x; s < LL::NP; s += Sh::NT) {
+            const int r = s / LL::PW, c = s - r * LL::PW;   // tile row r - 1, column c - 1
+            const int w = (r + 1) * LL::WW + c + 1;
+            const R mx = maxr<R>(maxr<R>(sx[w - LL::WW], sx[w]), sx[w + LL::WW]);
+            const R my = maxr<R>(maxr<R>(sy[w - 1], sy[w]), sy[w + 1]);
+            if (c >= 1 && c <= TJ) Lm->kx[r * TJ + c - 1] = C::div(R(2) * C::madd(lam_x, mx, lam_y * my), mx);
+            if (r >= 1 && r <= TI) Lm->ky[(r - 1) * LL::PW + c] = C::div(R(2) * C::madd(lam_y, my, lam_x * mx), my);
+        }
+    } else {
+        for (int s = threadIdx.x; s < NS; s += Sh::NT) {
+            const R rho = S[s], mx = S[NS + s], my = S[2 * NS + s], E = S[3 * NS + s];
+            const R ir = C::div(R(1), rho);
+            const R ux = mx * ir, uy = my * ir;
+            const R p = gm1 * (E - C::half * (mx * ux + my * uy));
+            const R cs = C::sqrt_((g * p) * ir);
+            Lm->sx[s] = (ux < R(0) ? -ux : ux) + cs;
+            Lm->sy[s] = (uy < R(0) ? -uy : uy) + cs;
+            Lm->ir[s] = ir;
+            Lm->p[s] = p;
+        }
+    }
+    __syncthreads();
+}
+
+// The LDS indices of a face's two cells in the limiter's 1/rho and pressure
+// planes (pL, pR) and its share factors (kL, kR).  cL: the left cell's index
+// in the state planes; s: the face's index among the tile's faces, x-faces
+// [0, NXF), then y-faces; lam_x, lam_y: dt/dx, dt/dy.
+template <typename R, typename Sh>
+__device__ __forceinline__ void limit_cells(const LimitLDS<R, Sh>& Lm, bool xf, int s, int cL, R lam_x, R lam_y,
+                                            int& pL, int& pR, R& kL, R& kR) {
+    constexpr int TJ = Sh::TJ, SC = Sh::SC;
+    if constexpr (sizeof(R) == 8) {
+        constexpr int PW = LimitCellK<R, Sh>::PW;
+        int kl;
+        if (xf) {
+            const int fr = s / TJ, jl = s - fr * TJ;
+            pL = fr * PW + jl + 1;
+            pR = pL + PW;
+            kl = s;
+            kL = Lm.kx[kl];
+            kR = Lm.kx[kl + TJ];
+        } else {
+            const int t = s - Sh::NXF, il = t / (TJ + 1), gl = t - il * (TJ + 1);
+            pL = (il + 1) * PW + gl;
+            pR = pL + 1;
+            kl = t + il;
+            kL = Lm.ky[kl];
+            kR = Lm.ky[kl + 1];
+        }
+    } else {
+        using C = K<R>;
+        const int dn = xf ? SC : 1, dtg = xf ? 1 : SC;
+        const R* sn = xf ? Lm.sx : Lm.sy;
+        const R* st = xf ? Lm.sy : Lm.sx;
+        const R ln = xf ? lam_x : lam_y, lt = xf ? lam_y : lam_x;
+        pL = cL;
+        pR = cL + dn;
+        const R anL = maxr<R>(maxr<R>(sn[pL - dn], sn[pL]), sn[pL + dn]);
+        const R atL = maxr<R>(maxr<R>(st[pL - dtg], st[pL]), st[pL + dtg]);
+        const R anR = maxr<R>(maxr<R>(sn[pR - dn], sn[pR]), sn[pR + dn]);
+        const R atR = maxr<R>(maxr<R>(st[pR - dtg], st[pR]), st[pR + dtg]);
+        kL = C::div(R(2) * C::madd(ln, anL, lt * atL), anL);
+        kR = C::div(R(2) * C::madd(ln, anR, lt * atR), anR);
+    }
+}
+
+// cL, cR: LDS cells left and right of the face; pL, pR: the same cells in the
+// 1/rho and pressure planes; kL, kR: their share factors
+template <typename R, typename Sh>
+__device__ __forceinline__ void limit_face(const R* __restrict__ S, int ns, const LimitLDS<R, Sh>& Lm, int cL, int cR,
+                                           int pL, int pR, R kL, R kR, int nrm, int tng, R g, R fl[4]) {
+    using C = K<R>;
     const R gm1 = g - R(1);
     const R UL[4] = {S[cL], S[nrm * ns + cL], S[tng * ns + cL], S[3 * ns + cL]};
     const R UR[4] = {S[cR], S[nrm * ns + cR], S[tng * ns + cR], S[3 * ns + cR]};
+    const R irL = Lm.ir[pL], prL = Lm.p[pL], irR = Lm.ir[pR], prR = Lm.p[pR];
     R FL[4], FR[4];
-    euler_flux_n<R>(UL[1], UL[2], UL[3], Lm.ir[cL], Lm.p[cL], FL);
-    euler_flux_n<R>(UR[1], UR[2], UR[3], Lm.ir[cR], Lm.p[cR], FR);
-    const R anL = maxr<R>(maxr<R>(sn[cL - dn], sn[cL]), sn[cL + dn]);
-    const R atL = maxr<R>(maxr<R>(st[cL - dtg], st[cL]), st[cL + dtg]);
-    const R anR = maxr<R>(maxr<R>(sn[cR - dn], sn[cR]), sn[cR + dn]);
-    const R atR = maxr<R>(maxr<R>(st[cR - dtg], st[cR]), st[cR + dtg]);
-    const R kL = C::div(R(2) * C::madd(ln, anL, lt * atL), anL);
-    const R kR = C::div(R(2) * C::madd(ln, anR, lt * atR), anR);
+    euler_flux_n<R>(UL[1], UL[2], UL[3], irL, prL, FL);
+    euler_flux_n<R>(UR[1], UR[2], UR[3], irR, prR, FR);
     // Both cells' shares with the face's own flux, U -+ k (F - F(U)): when both
     // keep their density and pressure at or above a floor that every limited
     // share also respects, theta = 1 and the flux stays as it is (nearly every
@@ -719,11 +836,22 @@ __device__ __forceinline__ void limit_face(const R* __restrict__ S, int ns, cons
         accL = HL[0] >= eps && rho_p(HL[0], HL[1], HL[2], HL[3], gm1) >= eps * HL[0];
         accR = HR[0] >= eps && rho_p(HR[0], HR[1], HR[2], HR[3], gm1) >= eps * HR[0];
     } else {
-        accL = HL[0] >= R(1e-3) * UL[0] && pressure_n<R>(HL[0], HL[1], HL[2], HL[3], gm1) >= R(1e-3) * Lm.p[cL];
-        accR = HR[0] >= R(1e-3) * UR[0] && pressure_n<R>(HR[0], HR[1], HR[2], HR[3], gm1) >= R(1e-3) * Lm.p[cR];
+        accL = HL[0] >= R(1e-3) * UL[0] && pressure_n<R>(HL[0], HL[1], HL[2], HL[3], gm1) >= R(1e-3) * prL;
+        accR = HR[0] >= R(1e-3) * UR[0] && pressure_n<R>(HR[0], HR[1], HR[2], HR[3], gm1) >= R(1e-3) * prR;
     }
     if (accL && accR) return;
-    const R a = maxr<R>(sn[cL], sn[cR]);
+    // |u_n| + c of both cells: FP32 keeps them, FP64 computes them again as
+    // limit_stage did
+    R snL, snR;
+    if constexpr (sizeof(R) == 8) {
+        const R unL = UL[1] * irL, unR = UR[1] * irR;
+        snL = (unL < R(0) ? -unL : unL) + C::sqrt_((g * prL) * irL);
+        snR = (unR < R(0) ? -unR : unR) + C::sqrt_((g * prR) * irR);
+    } else {
+        snL = (nrm == 1 ? Lm.sx : Lm.sy)[pL];
+        snR = (nrm == 1 ? Lm.sx : Lm.sy)[pR];
+    }
+    const R a = maxr<R>(snL, snR);
     R Flf[4], D[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -878,9 +1006,12 @@ __device__ __forceinline__ void tile_face(const StageArgs& a, int s, int i0, int
     R fl[4];
     face_flux<R, FAST>(S, NS, cell, nrm, tng, g, a.positivity, a.hll_bounds, fl);
     if constexpr (LIMIT) {
-        if (a.positivity == 2)
-            limit_face<R, Sh>(S, NS, *Lm, cell[2], cell[3], nrm, tng, xf ? SC : 1, xf ? 1 : SC,
-                              xf ? lam_x : lam_y, xf ? lam_y : lam_x, g, fl);
+        if (a.positivity == 2) {
+            int pL, pR;
+            R kL, kR;
+            limit_cells<R, Sh>(*Lm, xf, s, cell[2], lam_x, lam_y, pL, pR, kL, kR);
+            limit_face<R, Sh>(S, NS, *Lm, cell[2], cell[3], pL, pR, kL, kR, nrm, tng, g, fl);
+        }
     }
     if constexpr (VISC) {   // net flux = inviscid - viscous (mass: minus zero)
         R fv[3];
@@ -956,8 +1087,12 @@ __device__ __forceinline__ void tile_face_pair(const StageArgs& a, int s, int i0
     riemann_flux_n<R, 2>(L, Rr, cl, cr, g, a.positivity, a.hll_bounds, fl);
     if constexpr (LIMIT) {
         if (a.positivity == 2) {
-            limit_face<R, Sh>(S, NS, *Lm, cell[0][2], cell[0][3], 1, 2, SC, 1, lam_x, lam_y, g, fl[0]);
-            limit_face<R, Sh>(S, NS, *Lm, cell[1][2], cell[1][3], 2, 1, 1, SC, lam_y, lam_x, g, fl[1]);
+            int pL[2], pR[2];
+            R kL[2], kR[2];
+            limit_cells<R, Sh>(*Lm, true, s, cell[0][2], lam_x, lam_y, pL[0], pR[0], kL[0], kR[0]);
+            limit_cells<R, Sh>(*Lm, false, NXF + sy, cell[1][2], lam_x, lam_y, pL[1], pR[1], kL[1], kR[1]);
+            limit_face<R, Sh>(S, NS, *Lm, cell[0][2], cell[0][3], pL[0], pR[0], kL[0], kR[0], 1, 2, g, fl[0]);
+            limit_face<R, Sh>(S, NS, *Lm, cell[1][2], cell[1][3], pL[1], pR[1], kL[1], kR[1], 2, 1, g, fl[1]);
         }
     }
     F[s] = fl[0][0];
@@ -997,8 +1132,8 @@ __device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, d
     constexpr int NXF = Sh::NXF, NF = Sh::NXF + Sh::NYF;
     const int tid = threadIdx.x;
     const R g = (R)a.gamma;
-    // Lax-Friedrichs shares of the limiter: dt/dx and dt/dy (x/d == x*(1/d)
-    // bit for bit when d is a power of two)
+    // the limiter's dt/dx and dt/dy (x/d == x*(1/d) bit for bit when d is a
+    // power of two); the FP32 limiter's share factors are per face
     const R lam_x = (R)(a.dx_pow2 ? dt * a.inv_dx : dt / a.dx);
     const R lam_y = (R)(a.dy_pow2 ? dt * a.inv_dy : dt / a.dy);
     if constexpr (SPLIT && Sh::DUAL) {
@@ -1168,20 +1303,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
 
     if constexpr (LIMIT) {
         if (a.positivity == 2) {
-            // |u| + c and |v| + c of every staged cell
-            const R g = (R)a.gamma, gm1 = g - R(1);
-            for (int s = tid; s < SR * SC; s += BLOCK) {
-                const R rho = S[s], mx = S[NS + s], my = S[2 * NS + s], E = S[3 * NS + s];
-                const R ir = K<R>::div(R(1), rho);
-                const R ux = mx * ir, uy = my * ir;
-                const R p = gm1 * (E - K<R>::half * (mx * ux + my * uy));
-                const R c = K<R>::sqrt_((g * p) * ir);
-                Lm->sx[s] = (ux < R(0) ? -ux : ux) + c;
-                Lm->sy[s] = (uy < R(0) ? -uy : uy) + c;
-                Lm->ir[s] = ir;
-                Lm->p[s] = p;
-            }
-            __syncthreads();
+            // dt/dx and dt/dy (x/d == x*(1/d) bit for bit when d is a power of two)
+            const R lam_x = (R)(a.dx_pow2 ? dt * a.inv_dx : dt / a.dx);
+            const R lam_y = (R)(a.dy_pow2 ? dt * a.inv_dy : dt / a.dy);
+            limit_stage<R, Sh>(S, F, Lm, (R)a.gamma, lam_x, lam_y);
         }
     }
 
@@ -1289,11 +1414,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
 // workgroup's LDS (160 KB per CU) allows.
 template <typename R, bool VISC, bool LIMIT, typename Sh> struct Occupancy {
     // the single-solve FP64 limiter kernel keeps one flux plane-set of x-faces
-    // (HALF): 30 KB instead of 38 KB of LDS per 15 x 16 tile, five workgroups
-    // per CU instead of four
+    // (HALF): 31.5 KB instead of 39.5 KB of LDS per 15 x 16 tile, five
+    // workgroups per CU instead of four
     static constexpr bool half_flux = sizeof(R) == 8 && LIMIT && !VISC && !Sh::DUAL;
     static constexpr int want = VISC ? (sizeof(R) == 8 ? 3 : 6)
-                              : Sh::DUAL ? 4
+                              : Sh::DUAL ? ((LIMIT && sizeof(R) == 8) ? 3 : 4)
                               : LIMIT ? (sizeof(R) == 8 ? 5 : 6) : (sizeof(R) == 8 ? 5 : 6);
     static constexpr int extra = VISC ? (int)sizeof(ViscLDS<R, Sh>) : LIMIT ? (int)sizeof(LimitLDS<R, Sh>) : 0;
     static constexpr int lds = (int)sizeof(R) * (Sh::SR * Sh::SC * 4 + Sh::template flux_plane<half_flux>() * 4) +
