@@ -142,12 +142,57 @@ template <typename R, typename Sh> struct ViscLDS {
 // ---------------------------------------------------------------------------
 // scalar building blocks (templated on the reconstruction/flux type)
 
+// FP64 division and square root as the compiler's IEEE sequences compute them
+// on gfx950, without their range handling, for operands where that handling
+// changes nothing -- the same operations on the same values, so the same
+// (correctly rounded) results.
+//
+// Division a/b: v_rcp_f64 and two Newton steps give r ~ 1/b, then q = a*r and
+// RN(q + (a - b q) r); around it the compiler's sequence scales the operands
+// (v_div_scale) only when b or 1/b is not normal, a/b is not normal,
+// exponent(a) - exponent(b) >= 768 or |a| < 2^-969, and fixes up (v_div_fixup)
+// only zeros, infinities and NaNs.  Taking the residual negated, b q - a, a
+// zero numerator over b > 0 gives IEEE's signed zero as well (a = -0: q = -0,
+// b q - a = +0, and -(+0) r + q = -0).
+// Square root: v_rsq_f64 and the Newton steps of the compiler's sequence,
+// which scales x by 2^256 only below 2^-767 and returns x itself for zero and
+// infinity.
+__device__ __forceinline__ double rcp64(double b) {
+    double r = __builtin_amdgcn_rcp(b);
+    r = __builtin_fma(r, __builtin_fma(-b, r, 1.0), r);
+    return __builtin_fma(r, __builtin_fma(-b, r, 1.0), r);
+}
+__device__ __forceinline__ double div64(double a, double b, double r) {
+    const double q = a * r;
+    const double e = __builtin_fma(b, q, -a);
+    return __builtin_fma(-e, r, q);
+}
+__device__ __forceinline__ double sqrt64_core(double x) {
+    const double r = __builtin_amdgcn_rsq(x);
+    double g = x * r, h = r * 0.5;
+    const double e = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, e, g);
+    h = __builtin_fma(h, e, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+}
+// IEEE sqrt: the short form on [2^-767, 2^1000] (nearly every value), else
+// the compiler's full sequence
+__device__ __forceinline__ double sqrt64(double x) {
+    double y;
+    if (x >= 0x1p-767 && x <= 0x1p+1000) y = sqrt64_core(x);
+    else y = __builtin_sqrt(x);
+    return y;
+}
+
 template <typename R> struct K;
 template <> struct K<double> {
     static constexpr double eps = 1e-2, c1312 = 13.0 / 12.0, q = 0.25;
     static constexpr double d0 = 0.1, d1 = 0.6, d2 = 0.3, half = 0.5;
     __device__ static double div(double a, double b) { return a / b; }
-    __device__ static double sqrt_(double a) { return __builtin_sqrt(a); }
+    __device__ static double sqrt_(double a) { return sqrt64(a); }
     // a*b + c rounded twice, as written in the reference (no contraction)
     __device__ static double madd(double a, double b, double c) { return a * b + c; }
 };
@@ -314,13 +359,42 @@ __device__ __forceinline__ void state_terms(R r, R n, R t, R E, R g, R gm1, R& p
     gp = C::div(g * p, r);
 }
 
+// state_terms with one reciprocal of the density for its five divisions
+// (div64).  In range when r in [2^-150, 2^150], n and t zero or of magnitude
+// in [2^-200, 2^150], E zero or in [2^-500, 2^450] and (checked once per
+// launch, exact_state_gamma) g in [2^-10, 2^10], g - 1 zero or of magnitude
+// in [2^-60, 2^10]: then n, n^2, n t and (n^2 + t^2)/2 are zero or in
+// [2^-401, 2^301], their quotients by r in [2^-551, 2^451]; E - (n^2+t^2)/(2r),
+// a difference of multiples of 2^-603, is zero or in [2^-603, 2^452], so g p
+// is zero or in [2^-673, 2^471] and g p / r normal: every division meets
+// div64's conditions.
+__device__ __forceinline__ bool exact_state_ok(double r, double n, double t, double E) {
+    const double an = __builtin_fabs(n), at = __builtin_fabs(t), aE = __builtin_fabs(E);
+    return r >= 0x1p-150 && r <= 0x1p+150 && an <= 0x1p+150 && (n == 0.0 || an >= 0x1p-200) && at <= 0x1p+150 &&
+           (t == 0.0 || at >= 0x1p-200) && aE <= 0x1p+450 && (E == 0.0 || aE >= 0x1p-500);
+}
+__host__ __device__ inline bool exact_state_gamma(double g) {
+    const double gm1 = g - 1.0, a = gm1 < 0.0 ? -gm1 : gm1;
+    return g >= 0x1p-10 && g <= 0x1p+10 && (gm1 == 0.0 || (a >= 0x1p-60 && a <= 0x1p+10));
+}
+__device__ __forceinline__ void state_terms_exact(double r, double n, double t, double E, double g, double gm1,
+                                                  double& p, double& un, double& nn, double& nt, double& gp) {
+    const double ir = rcp64(r);
+    const double n2 = n * n;
+    p = gm1 * (E - div64(0.5 * (n2 + t * t), r, ir));
+    un = div64(n, r, ir);
+    nn = div64(n2, r, ir);
+    nt = div64(n * t, r, ir);
+    gp = div64(g * p, r, ir);
+}
+
 template <typename R>
 __device__ __forceinline__ void riemann_flux(const R L[4], const R Rr[4], const R cl[4], const R cr[4], R g,
-                                             int positivity, int hll_bounds, R out[4]);
+                                             int positivity, int hll_bounds, bool xg, R out[4]);
 
 template <typename R, bool FAST>
 __device__ __forceinline__ void face_flux(const R* __restrict__ S, int ns, const int cell[6], int nrm, int tng,
-                                          R g, int positivity, int hll_bounds, R out[4]) {
+                                          R g, int positivity, int hll_bounds, bool xg, R out[4]) {
     R L[4], Rr[4], cl[4], cr[4];
     const int comp[4] = {0, nrm, tng, 3};
 #pragma unroll
@@ -333,7 +407,7 @@ __device__ __forceinline__ void face_flux(const R* __restrict__ S, int ns, const
         cl[q] = v[2];
         cr[q] = v[3];
     }
-    riemann_flux<R>(L, Rr, cl, cr, g, positivity, hll_bounds, out);
+    riemann_flux<R>(L, Rr, cl, cr, g, positivity, hll_bounds, xg, out);
 }
 
 // Pressures, the positivity fallback, Euler fluxes and the HLL flux of N
@@ -342,7 +416,7 @@ __device__ __forceinline__ void face_flux(const R* __restrict__ S, int ns, const
 // one, so N > 1 gives the scheduler N independent dependency chains.
 template <typename R, int N>
 __device__ __forceinline__ void riemann_flux_n(const R (*L)[4], const R (*Rr)[4], const R (*cl)[4], const R (*cr)[4],
-                                               R g, int positivity, int hll_bounds, R (*out)[4]) {
+                                               R g, int positivity, int hll_bounds, bool xg, R (*out)[4]) {
     using C = K<R>;
     const R gm1 = g - R(1);
     R rL[N], nL[N], tL[N], EL[N], rR[N], nR[N], tR[N], ER[N];
@@ -353,8 +427,19 @@ __device__ __forceinline__ void riemann_flux_n(const R (*L)[4], const R (*Rr)[4]
         rR[f] = Rr[f][0]; nR[f] = Rr[f][1]; tR[f] = Rr[f][2]; ER[f] = Rr[f][3];
         // pressure and the state's divisions by its density (EulerPhase.cpp:132,
         // EulerMod.cpp:35-50, RiemmanSolverHLL.cpp:18-23)
-        state_terms<R>(rL[f], nL[f], tL[f], EL[f], g, gm1, pL[f], unL[f], nnL[f], ntL[f], gpL[f]);
-        state_terms<R>(rR[f], nR[f], tR[f], ER[f], g, gm1, pR[f], unR[f], nnR[f], ntR[f], gpR[f]);
+        // (one face at a time only: two faces' both forms spill)
+        bool exact = false;
+        if constexpr (sizeof(R) == 8 && N == 1)
+            exact = xg && exact_state_ok(rL[f], nL[f], tL[f], EL[f]) && exact_state_ok(rR[f], nR[f], tR[f], ER[f]);
+        if (exact) {
+            if constexpr (sizeof(R) == 8 && N == 1) {
+                state_terms_exact(rL[f], nL[f], tL[f], EL[f], g, gm1, pL[f], unL[f], nnL[f], ntL[f], gpL[f]);
+                state_terms_exact(rR[f], nR[f], tR[f], ER[f], g, gm1, pR[f], unR[f], nnR[f], ntR[f], gpR[f]);
+            }
+        } else {
+            state_terms<R>(rL[f], nL[f], tL[f], EL[f], g, gm1, pL[f], unL[f], nnL[f], ntL[f], gpL[f]);
+            state_terms<R>(rR[f], nR[f], tR[f], ER[f], g, gm1, pR[f], unR[f], nnR[f], ntR[f], gpR[f]);
+        }
     }
     if (positivity) {
 #pragma unroll
@@ -412,10 +497,10 @@ __device__ __forceinline__ void riemann_flux_n(const R (*L)[4], const R (*Rr)[4]
 
 template <typename R>
 __device__ __forceinline__ void riemann_flux(const R L[4], const R Rr[4], const R cl[4], const R cr[4], R g,
-                                             int positivity, int hll_bounds, R out[4]) {
+                                             int positivity, int hll_bounds, bool xg, R out[4]) {
     R o[1][4];
     riemann_flux_n<R, 1>((const R(*)[4])L, (const R(*)[4])Rr, (const R(*)[4])cl, (const R(*)[4])cr, g, positivity,
-                         hll_bounds, o);
+                         hll_bounds, xg, o);
 #pragma unroll
     for (int q = 0; q < 4; ++q) out[q] = o[0][q];
 }
@@ -534,7 +619,7 @@ __device__ __forceinline__ void ghost_value(const double* __restrict__ U, long l
 __device__ __forceinline__ double cell_speed(double rho, double mu, double mv, double E, double gamma, double dx,
                                              double dy, int hll_bounds) {
     const double p = (gamma - 1) * (E - 0.5 * (mu * mu + mv * mv) / rho);
-    const double cs = __builtin_sqrt(gamma * p / rho);
+    const double cs = sqrt64(gamma * p / rho);
     const double u = mu / rho, v = mv / rho;
     double ax = __builtin_fabs(u), ay = __builtin_fabs(v);
     if (hll_bounds == 0) {
@@ -556,7 +641,7 @@ __device__ __forceinline__ double cell_speed_bulk(double rho, double mu, double 
     ir = __builtin_fma(ir, __builtin_fma(-rho, ir, 1.0), ir);
     const double u = mu * ir, v = mv * ir;
     const double p = (gamma - 1) * (E - 0.5 * (mu * u + mv * v));
-    const double cs = __builtin_sqrt((gamma * p) * ir);
+    const double cs = sqrt64((gamma * p) * ir);
     double ax = __builtin_fabs(u), ay = __builtin_fabs(v);
     if (hll_bounds == 0) {
         if (ax < __builtin_fabs(mu)) ax = __builtin_fabs(mu);
@@ -707,38 +792,53 @@ __device__ __forceinline__ void euler_flux_n(R n, R t, R E, R ir, R p, R F[4]) {
     F[3] = (E + p) * un;
 }
 
-// |u| + c and |v| + c of the cells around a tile (the |u_n| + c of the
-// limiter) and 1/rho and pressure of the cells next to its faces; FP64 also
-// the share factors k = 2 (ln an + lt at) / an of those cells (an, at: the
-// largest |u_n| + c, |u_t| + c over the cell and its two neighbours along n,
-// t; ln, lt: dt/h along them).  W: scratch room for the FP64 wave speeds.
+// The limiter's values of staged cell s (tile row r, column c; state rho, mx,
+// my, E), computed as the cell is staged:
+// |u| + c and |v| + c (the |u_n| + c of the limiter) and, for the cells next
+// to a tile face, 1/rho and the pressure.  FP64 keeps the wave speeds of rows
+// and columns -2..TI+1 / -2..TJ+1 in W (the flux planes, not yet written).
 template <typename R, typename Sh>
-__device__ __forceinline__ void limit_stage(const R* __restrict__ S, R* __restrict__ W, LimitLDS<R, Sh>* Lm, R g,
-                                            R lam_x, R lam_y) {
+__device__ __forceinline__ void limit_cell(int s, int r, int c, R rho, R mx, R my, R E, R g, R* __restrict__ W,
+                                           LimitLDS<R, Sh>* Lm) {
     using C = K<R>;
-    constexpr int TI = Sh::TI, TJ = Sh::TJ, SC = Sh::SC, NS = Sh::SR * Sh::SC;
-    const R gm1 = g - R(1);
+    constexpr int TI = Sh::TI, TJ = Sh::TJ;
+    const R ir = C::div(R(1), rho);
+    const R ux = mx * ir, uy = my * ir;
+    const R p = (g - R(1)) * (E - C::half * (mx * ux + my * uy));
+    const R cs = C::sqrt_((g * p) * ir);
+    const R sx = (ux < R(0) ? -ux : ux) + cs, sy = (uy < R(0) ? -uy : uy) + cs;
     if constexpr (sizeof(R) == 8) {
         using LL = LimitCellK<R, Sh>;
-        R* sx = W;
-        R* sy = W + LL::NW;
-        for (int s = threadIdx.x; s < LL::NW; s += Sh::NT) {
-            const int r = s / LL::WW, c = s - r * LL::WW;   // tile row r - 2, column c - 2
-            const int e = (r + HALO - 2) * SC + c + HALO - 2;
-            const R rho = S[e], mx = S[NS + e], my = S[2 * NS + e], E = S[3 * NS + e];
-            const R ir = C::div(R(1), rho);
-            const R ux = mx * ir, uy = my * ir;
-            const R p = gm1 * (E - C::half * (mx * ux + my * uy));
-            const R cs = C::sqrt_((g * p) * ir);
-            sx[s] = (ux < R(0) ? -ux : ux) + cs;
-            sy[s] = (uy < R(0) ? -uy : uy) + cs;
-            if (r >= 1 && r <= TI + 2 && c >= 1 && c <= TJ + 2) {
-                const int q = (r - 1) * LL::PW + c - 1;
+        if (r >= -2 && r <= TI + 1 && c >= -2 && c <= TJ + 1) {
+            const int w = (r + 2) * LL::WW + c + 2;
+            W[w] = sx;
+            W[LL::NW + w] = sy;
+            if (r >= -1 && r <= TI && c >= -1 && c <= TJ) {
+                const int q = (r + 1) * LL::PW + c + 1;
                 Lm->ir[q] = ir;
                 Lm->p[q] = p;
             }
         }
-        __syncthreads();
+    } else {
+        Lm->sx[s] = sx;
+        Lm->sy[s] = sy;
+        Lm->ir[s] = ir;
+        Lm->p[s] = p;
+    }
+}
+
+// FP64: the share factors k = 2 (ln an + lt at) / an of the cells next to a
+// tile face (an, at: the largest |u_n| + c, |u_t| + c over the cell and its
+// two neighbours along n, t; ln, lt: dt/h along them), from the wave speeds
+// in W, once the tile is staged
+template <typename R, typename Sh>
+__device__ __forceinline__ void limit_share(const R* __restrict__ W, LimitLDS<R, Sh>* Lm, R lam_x, R lam_y) {
+    if constexpr (sizeof(R) == 8) {
+        using LL = LimitCellK<R, Sh>;
+        using C = K<R>;
+        constexpr int TI = Sh::TI, TJ = Sh::TJ;
+        const R* sx = W;
+        const R* sy = W + LL::NW;
         for (int s = threadIdx.x; s < LL::NP; s += Sh::NT) {
             const int r = s / LL::PW, c = s - r * LL::PW;   // tile row r - 1, column c - 1
             const int w = (r + 1) * LL::WW + c + 1;
@@ -747,20 +847,8 @@ __device__ __forceinline__ void limit_stage(const R* __restrict__ S, R* __restri
             if (c >= 1 && c <= TJ) Lm->kx[r * TJ + c - 1] = C::div(R(2) * C::madd(lam_x, mx, lam_y * my), mx);
             if (r >= 1 && r <= TI) Lm->ky[(r - 1) * LL::PW + c] = C::div(R(2) * C::madd(lam_y, my, lam_x * mx), my);
         }
-    } else {
-        for (int s = threadIdx.x; s < NS; s += Sh::NT) {
-            const R rho = S[s], mx = S[NS + s], my = S[2 * NS + s], E = S[3 * NS + s];
-            const R ir = C::div(R(1), rho);
-            const R ux = mx * ir, uy = my * ir;
-            const R p = gm1 * (E - C::half * (mx * ux + my * uy));
-            const R cs = C::sqrt_((g * p) * ir);
-            Lm->sx[s] = (ux < R(0) ? -ux : ux) + cs;
-            Lm->sy[s] = (uy < R(0) ? -uy : uy) + cs;
-            Lm->ir[s] = ir;
-            Lm->p[s] = p;
-        }
+        __syncthreads();
     }
-    __syncthreads();
 }
 
 // The LDS indices of a face's two cells in the limiter's 1/rho and pressure
@@ -968,7 +1056,7 @@ __device__ __forceinline__ void viscous_flux(const ViscLDS<R, Sh>& V, const int 
 // (0 or n of a non-periodic axis), so every stencil is the plain 6 cells and
 // its LDS offsets are constants.
 template <typename R, bool VISC, bool LIMIT, bool FAST, typename Sh, int DIR, bool HALF, bool REG = false>
-__device__ __forceinline__ void tile_face(const StageArgs& a, int s, int i0, int j0, R g, R lam_x, R lam_y,
+__device__ __forceinline__ void tile_face(const StageArgs& a, int s, int i0, int j0, R g, bool xg, R lam_x, R lam_y,
                                           const R* S, R* F, const ViscLDS<R, Sh>* V, const LimitLDS<R, Sh>* Lm) {
     constexpr int TJ = Sh::TJ, SC = Sh::SC, NXF = Sh::NXF;
     constexpr int NS = Sh::SR * Sh::SC, NF = Sh::template flux_plane<HALF>();   // LDS plane lengths
@@ -1004,7 +1092,7 @@ __device__ __forceinline__ void tile_face(const StageArgs& a, int s, int i0, int
         }
     }
     R fl[4];
-    face_flux<R, FAST>(S, NS, cell, nrm, tng, g, a.positivity, a.hll_bounds, fl);
+    face_flux<R, FAST>(S, NS, cell, nrm, tng, g, a.positivity, a.hll_bounds, xg, fl);
     if constexpr (LIMIT) {
         if (a.positivity == 2) {
             int pL, pR;
@@ -1037,8 +1125,8 @@ __device__ __forceinline__ void tile_face(const StageArgs& a, int s, int i0, int
 // The tile has one y-face fewer than x-faces: the last thread computes a copy
 // of the y-face before it and does not store it.
 template <typename R, bool LIMIT, bool FAST, typename Sh>
-__device__ __forceinline__ void tile_face_pair(const StageArgs& a, int s, int i0, int j0, R g, R lam_x, R lam_y,
-                                               const R* S, R* F, const LimitLDS<R, Sh>* Lm) {
+__device__ __forceinline__ void tile_face_pair(const StageArgs& a, int s, int i0, int j0, R g, bool xg, R lam_x,
+                                               R lam_y, const R* S, R* F, const LimitLDS<R, Sh>* Lm) {
     constexpr int TJ = Sh::TJ, SC = Sh::SC, NXF = Sh::NXF, NYF = Sh::NYF;
     constexpr int NS = Sh::SR * Sh::SC, NF = NXF + NYF;
     const int sy = s < NYF ? s : NYF - 1;
@@ -1084,7 +1172,7 @@ __device__ __forceinline__ void tile_face_pair(const StageArgs& a, int s, int i0
         }
     }
     R fl[2][4];
-    riemann_flux_n<R, 2>(L, Rr, cl, cr, g, a.positivity, a.hll_bounds, fl);
+    riemann_flux_n<R, 2>(L, Rr, cl, cr, g, a.positivity, a.hll_bounds, xg, fl);
     if constexpr (LIMIT) {
         if (a.positivity == 2) {
             int pL[2], pR[2];
@@ -1132,13 +1220,16 @@ __device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, d
     constexpr int NXF = Sh::NXF, NF = Sh::NXF + Sh::NYF;
     const int tid = threadIdx.x;
     const R g = (R)a.gamma;
+    // the FP64 face states' divisions by their density with one reciprocal
+    // where in range (state_terms_exact; the gamma condition checked here)
+    const bool xg = sizeof(R) == 8 && exact_state_gamma(a.gamma);
     // the limiter's dt/dx and dt/dy (x/d == x*(1/d) bit for bit when d is a
     // power of two); the FP32 limiter's share factors are per face
     const R lam_x = (R)(a.dx_pow2 ? dt * a.inv_dx : dt / a.dx);
     const R lam_y = (R)(a.dy_pow2 ? dt * a.inv_dy : dt / a.dy);
     if constexpr (SPLIT && Sh::DUAL) {
         static_assert(!HALF && !VISC, "both directions' fluxes are live at once; inviscid only");
-        for (int s = tid; s < NXF; s += Sh::NT) tile_face_pair<R, LIMIT, FAST, Sh>(a, s, i0, j0, g, lam_x, lam_y, S, F, Lm);
+        for (int s = tid; s < NXF; s += Sh::NT) tile_face_pair<R, LIMIT, FAST, Sh>(a, s, i0, j0, g, xg, lam_x, lam_y, S, F, Lm);
     } else if constexpr (SPLIT) {
         // a direction whose tile faces include no degenerate boundary face
         // (the fast division forms only: the IEEE form serves rare tiles)
@@ -1146,10 +1237,10 @@ __device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, d
         const bool yreg = FAST && (a.bc[2] == BC_PERIODIC || (j0 > 0 && j0 + Sh::TJ < a.ny));
         if (xreg) {
             for (int s = tid; s < NXF; s += Sh::NT)
-                tile_face<R, VISC, LIMIT, FAST, Sh, 1, HALF, FAST>(a, s, i0, j0, g, lam_x, lam_y, S, F, V, Lm);
+                tile_face<R, VISC, LIMIT, FAST, Sh, 1, HALF, FAST>(a, s, i0, j0, g, xg, lam_x, lam_y, S, F, V, Lm);
         } else {
             for (int s = tid; s < NXF; s += Sh::NT)
-                tile_face<R, VISC, LIMIT, FAST, Sh, 1, HALF>(a, s, i0, j0, g, lam_x, lam_y, S, F, V, Lm);
+                tile_face<R, VISC, LIMIT, FAST, Sh, 1, HALF>(a, s, i0, j0, g, xg, lam_x, lam_y, S, F, V, Lm);
         }
         if constexpr (HALF) {
             __syncthreads();
@@ -1162,14 +1253,14 @@ __device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, d
         }
         if (yreg) {
             for (int s = NXF + tid; s < NF; s += Sh::NT)
-                tile_face<R, VISC, LIMIT, FAST, Sh, 2, HALF, FAST>(a, s, i0, j0, g, lam_x, lam_y, S, F, V, Lm);
+                tile_face<R, VISC, LIMIT, FAST, Sh, 2, HALF, FAST>(a, s, i0, j0, g, xg, lam_x, lam_y, S, F, V, Lm);
         } else {
             for (int s = NXF + tid; s < NF; s += Sh::NT)
-                tile_face<R, VISC, LIMIT, FAST, Sh, 2, HALF>(a, s, i0, j0, g, lam_x, lam_y, S, F, V, Lm);
+                tile_face<R, VISC, LIMIT, FAST, Sh, 2, HALF>(a, s, i0, j0, g, xg, lam_x, lam_y, S, F, V, Lm);
         }
     } else {
         for (int s = tid; s < NF; s += Sh::NT)
-            tile_face<R, VISC, LIMIT, FAST, Sh, 0, HALF>(a, s, i0, j0, g, lam_x, lam_y, S, F, V, Lm);
+            tile_face<R, VISC, LIMIT, FAST, Sh, 0, HALF>(a, s, i0, j0, g, xg, lam_x, lam_y, S, F, V, Lm);
     }
 }
 
@@ -1260,13 +1351,15 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
     } else {
         dt = *a.dt;
     }
-    int unsafe = 0;
+    bool unsafe = false;
 
     // ---- tile + halo -> LDS (corners are never read) ----
     const bool near_ib = a.tile_ib != nullptr && G(a.tile_ib)[tile];
     // a tile whose halo lies inside the grid reads its cells directly: no
     // boundary ghost, no periodic wrap (the loads fetch_cell makes there)
     const bool interior = i0 >= HALO && i0 + TI + HALO <= a.nx && j0 >= HALO && j0 + TJ + HALO <= a.ny;
+    // the limiter's per-cell values (positivity = 2) are taken as each cell is staged
+    const bool limit = LIMIT && a.positivity == 2;
     for (int s = tid; s < SR * SC; s += BLOCK) {
         const int r = s / SC, c = s - r * SC;
         const bool xs = (c >= HALO && c < HALO + TJ), ys = (r >= HALO && r < HALO + TI);
@@ -1293,12 +1386,16 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             S[q * NS + s] = (R)v[q];
-            if (sizeof(R) == 8) unsafe |= !fast_div_safe(v[q]);
+            if (sizeof(R) == 8) unsafe = unsafe || !fast_div_safe(v[q]);
+        }
+        if constexpr (LIMIT) {
+            if (limit)
+                limit_cell<R, Sh>(s, r - HALO, c - HALO, (R)v[0], (R)v[1], (R)v[2], (R)v[3], (R)a.gamma, F, Lm);
         }
     }
     // one barrier + OR over the tile: tiles outside the range of the fast
     // WENO5 division forms take the IEEE form (and are counted)
-    const bool tile_unsafe = __syncthreads_or(unsafe) != 0;
+    const bool tile_unsafe = __syncthreads_or(unsafe ? 1 : 0) != 0;
     if (tile_unsafe && tid == 0) atomicAdd(a.exact_count, 1);
 
     if constexpr (LIMIT) {
@@ -1306,7 +1403,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
             // dt/dx and dt/dy (x/d == x*(1/d) bit for bit when d is a power of two)
             const R lam_x = (R)(a.dx_pow2 ? dt * a.inv_dx : dt / a.dx);
             const R lam_y = (R)(a.dy_pow2 ? dt * a.inv_dy : dt / a.dy);
-            limit_stage<R, Sh>(S, F, Lm, (R)a.gamma, lam_x, lam_y);
+            limit_share<R, Sh>(F, Lm, lam_x, lam_y);
         }
     }
 
