@@ -4,9 +4,10 @@
 // a slot table of per-problem device scalars, the device array of per-problem
 // stage arguments (2 step parities x 3 sub-stages per slot), the device list
 // of (slot, tile) pairs the persistent stage kernel walks, one HIP stream and
-// a HIP graph of two RK steps.  The graph's kernel arguments are only device
+// HIP graphs of two and of eight RK steps (consecutive replays leave a few
+// microseconds between them).  The graphs' kernel arguments are only device
 // pointers and the launch shape, so admitting and retiring problems (which
-// rewrite the device arrays between replays) never invalidates it.
+// rewrite the device arrays between replays) never invalidates them.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -95,8 +96,10 @@ struct cfs_batch {
     int device = 0;                   // the device current at creation
     int parity = 0;                   // step parity of the next step
     bool dirty = false;               // tile list to rebuild before the next step
-    hipGraph_t graph = nullptr;
+    hipGraph_t graph = nullptr;            // two RK steps
     hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph8 = nullptr;           // eight RK steps
+    hipGraphExec_t graph8_exec = nullptr;
 };
 
 namespace {
@@ -163,8 +166,12 @@ int rebuild_tiles(cfs_batch* b) {
 void drop_graph(cfs_batch* b) {
     if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->graph8_exec) (void)hipGraphExecDestroy(b->graph8_exec);
+    if (b->graph8) (void)hipGraphDestroy(b->graph8);
     b->graph_exec = nullptr;
     b->graph = nullptr;
+    b->graph8_exec = nullptr;
+    b->graph8 = nullptr;
 }
 
 long long tiles_of(const cfs_batch* b, const cfs_config& c) { return tiles_for(c, b->bc.tile_shape); }
@@ -401,6 +408,24 @@ int32_t cfs_batch_admit(cfs_batch* b, const cfs_config* cfg, const double* U, co
     return CFS_OK;
 }
 
+// A graph of `steps` RK steps (even: the step parities alternate), starting at
+// parity 0.
+static int capture_steps(cfs_batch* b, int steps, hipGraph_t* graph, hipGraphExec_t* exec) {
+    HIP_OK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int crc = 0;
+    for (int k = 0; k < steps && !crc; ++k) crc = launch_batch_step(b, k & 1);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(b->stream, &g);
+    if (crc) {
+        if (g) (void)hipGraphDestroy(g);
+        return crc;
+    }
+    HIP_OK(ec);
+    *graph = g;
+    HIP_OK(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+    return CFS_OK;
+}
+
 int32_t cfs_batch_step(cfs_batch* b, int32_t nsteps) {
     if (!b || nsteps < 0) return CFS_E_ARG;
     int rc;
@@ -411,21 +436,12 @@ int32_t cfs_batch_step(cfs_batch* b, int32_t nsteps) {
         b->parity = 0;
         --n;
     }
+    if (n >= 8) {
+        if (!b->graph8_exec && (rc = capture_steps(b, 8, &b->graph8, &b->graph8_exec))) return rc;
+        for (; n >= 8; n -= 8) HIP_OK(hipGraphLaunch(b->graph8_exec, b->stream));
+    }
     if (n >= 2) {
-        if (!b->graph_exec) {
-            HIP_OK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-            int crc = launch_batch_step(b, 0);
-            if (!crc) crc = launch_batch_step(b, 1);
-            hipGraph_t g = nullptr;
-            const hipError_t ec = hipStreamEndCapture(b->stream, &g);
-            if (crc) {
-                if (g) (void)hipGraphDestroy(g);
-                return crc;
-            }
-            HIP_OK(ec);
-            b->graph = g;
-            HIP_OK(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0));
-        }
+        if (!b->graph_exec && (rc = capture_steps(b, 2, &b->graph, &b->graph_exec))) return rc;
         for (; n >= 2; n -= 2) HIP_OK(hipGraphLaunch(b->graph_exec, b->stream));
     }
     if (n == 1) {

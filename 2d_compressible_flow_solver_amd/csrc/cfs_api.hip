@@ -4,7 +4,7 @@
 // cell-type plane and immersed-boundary tables (plus a per-cell ghost index
 // and a per-tile "near the body" flag for the fused fill), a block of device
 // scalars (dt, time, CFL bounds, halt/status/step counters), scratch for the
-// checksum tree, one HIP stream and a cached HIP graph of two RK steps.
+// checksum tree, one HIP stream and a cached HIP graph of RK steps.
 // Nothing here falls back to a CPU path: every compute entry point launches
 // HIP kernels and fails with CFS_E_HIP when HIP does.
 #include <hip/hip_runtime.h>
@@ -42,6 +42,7 @@ struct GraphKey {
     bool use_cfl = false;
     double cfl = 0.0, t_end = 0.0;
     int step_parity = 0, stage_parity = 0;
+    int steps = 0;   // RK steps captured
 };
 
 }  // namespace
@@ -162,6 +163,12 @@ int launch_step(cfs_solver* s, bool timed, bool use_cfl, double cfl, double t_en
 
 // steps between two reads of the device halt flag in a CFL advance
 constexpr int HALT_POLL = 64;
+// RK steps per graph replay of a long CFL advance (even: the time and bound
+// slots alternate by step parity; a divisor of HALT_POLL).  Consecutive
+// replays leave a few microseconds between them, about 2% of a 400^2 step
+// with two steps per graph.
+constexpr int GRAPH_STEPS = 8;
+static_assert(GRAPH_STEPS % 2 == 0 && HALT_POLL % GRAPH_STEPS == 0, "graph steps");
 
 int read_halt(cfs_solver* s, bool* halted) {
     int h = 0;
@@ -430,7 +437,8 @@ int32_t cfs_advance(cfs_solver* s, int32_t nsteps, double dt, double cfl, double
         if (rc) return rc;
         int n = 0;
         if (!timed && nsteps >= 4) {
-            // two steps per graph (time and bound slots alternate by step parity)
+            // GRAPH_STEPS steps per graph (two for shorter advances)
+            const int gsteps = nsteps >= 4 * GRAPH_STEPS ? GRAPH_STEPS : 2;
             GraphKey key;
             key.valid = true;
             key.use_cfl = true;
@@ -438,16 +446,17 @@ int32_t cfs_advance(cfs_solver* s, int32_t nsteps, double dt, double cfl, double
             key.t_end = t_end;
             key.step_parity = s->step_parity;
             key.stage_parity = s->parity;
+            key.steps = gsteps;
             const bool same = s->graph_key.valid && s->graph_key.use_cfl == key.use_cfl &&
                               s->graph_key.cfl == key.cfl && s->graph_key.t_end == key.t_end &&
                               s->graph_key.step_parity == key.step_parity &&
-                              s->graph_key.stage_parity == key.stage_parity;
+                              s->graph_key.stage_parity == key.stage_parity && s->graph_key.steps == key.steps;
             const int counter0 = s->counter;
             if (!same) {
                 drop_graph(s);
                 HIP_OK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-                int crc = launch_step(s, false, true, cfl, t_end);
-                if (!crc) crc = launch_step(s, false, true, cfl, t_end);
+                int crc = 0;
+                for (int k = 0; k < gsteps && !crc; ++k) crc = launch_step(s, false, true, cfl, t_end);
                 hipGraph_t g = nullptr;
                 const hipError_t ec = hipStreamEndCapture(s->stream, &g);
                 if (crc) {
@@ -460,21 +469,21 @@ int32_t cfs_advance(cfs_solver* s, int32_t nsteps, double dt, double cfl, double
                 s->graph_key = key;
                 s->counter = counter0;   // capture launched nothing
             }
-            // the captured launches left step/stage parities and planes where two
-            // real steps leave them.  Every HALT_POLL steps the halt flag is read
-            // back: once the end time is reached (or the bound is not finite)
-            // the remaining steps would do nothing, so they are not launched.
-            for (; n + 2 <= nsteps; n += 2) {
+            // the captured launches left step/stage parities and planes where
+            // gsteps real steps leave them.  Every HALT_POLL steps the halt flag
+            // is read back: once the end time is reached (or the bound is not
+            // finite) the remaining steps would do nothing, so they are not launched.
+            for (; n + gsteps <= nsteps; n += gsteps) {
                 if (n > 0 && n % HALT_POLL == 0) {
                     bool halted = false;
                     if ((rc = read_halt(s, &halted))) return rc;
                     if (halted) break;
                 }
                 HIP_OK(hipGraphLaunch(s->graph_exec, s->stream));
-                s->counter += 2 * s->cfg.rk;
+                s->counter += gsteps * s->cfg.rk;
             }
             s->cur = 0;
-            if (n + 2 <= nsteps) n = nsteps;   // stopped at a halt
+            if (n + gsteps <= nsteps) n = nsteps;   // stopped at a halt
         }
         for (; n < nsteps; ++n) {
             if (n > 0 && n % HALT_POLL == 0) {
