@@ -1360,19 +1360,27 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
     const bool interior = i0 >= HALO && i0 + TI + HALO <= a.nx && j0 >= HALO && j0 + TJ + HALO <= a.ny;
     // the limiter's per-cell values (positivity = 2) are taken as each cell is staged
     const bool limit = LIMIT && a.positivity == 2;
-    for (int s = tid; s < SR * SC; s += BLOCK) {
+    // FP64: the cells a thread stages come two per pass (s and s + BLOCK), both
+    // read from memory before either is written to LDS, one memory latency per
+    // pass (FP32-bulk kernels spill with both cells live: one per pass)
+    constexpr int PER = sizeof(R) == 8 ? 2 : 1;
+    auto needed = [&](int s) {   // viscous cross terms and the limiter read the corners
         const int r = s / SC, c = s - r * SC;
-        const bool xs = (c >= HALO && c < HALO + TJ), ys = (r >= HALO && r < HALO + TI);
-        if (!VISC && !LIMIT && !(xs || ys)) continue;   // viscous cross terms and the limiter read the corners
-        double v[4];
-        long long cell;
+        return VISC || LIMIT || (c >= HALO && c < HALO + TJ) || (r >= HALO && r < HALO + TI);
+    };
+    auto load = [&](int s, double v[4]) -> long long {
+        const int r = s / SC, c = s - r * SC;
         if (interior) {
-            cell = (long long)(i0 - HALO + r) * a.ny + (j0 - HALO + c);
+            const long long cell = (long long)(i0 - HALO + r) * a.ny + (j0 - HALO + c);
 #pragma unroll
             for (int q = 0; q < 4; ++q) v[q] = G(a.in)[q * a.plane + cell];
-        } else {
-            cell = fetch_cell(a, i0 - HALO + r, j0 - HALO + c, v);
+            return cell;
         }
+        return fetch_cell(a, i0 - HALO + r, j0 - HALO + c, v);
+    };
+    auto store = [&](int s, double v[4], long long cell) {
+        const int r = s / SC, c = s - r * SC;
+        const bool xs = (c >= HALO && c < HALO + TJ), ys = (r >= HALO && r < HALO + TI);
         if (near_ib && cell >= 0 && G(a.type)[cell] == 2) {
             // immersed ghost: its value at the start of this stage, from its donors
             const int k = a.ig_index[cell];
@@ -1392,6 +1400,16 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
             if (limit)
                 limit_cell<R, Sh>(s, r - HALO, c - HALO, (R)v[0], (R)v[1], (R)v[2], (R)v[3], (R)a.gamma, F, Lm);
         }
+    };
+    for (int s0 = tid; s0 < SR * SC; s0 += PER * BLOCK) {
+        const int s1 = s0 + BLOCK;
+        const bool n0 = needed(s0), n1 = PER == 2 && s1 < SR * SC && needed(s1);
+        double v0[4], v1[4];
+        long long c0 = -1, c1 = -1;
+        if (n0) c0 = load(s0, v0);
+        if (n1) c1 = load(s1, v1);
+        if (n0) store(s0, v0, c0);
+        if (n1) store(s1, v1, c1);
     }
     // one barrier + OR over the tile: tiles outside the range of the fast
     // WENO5 division forms take the IEEE form (and are counted)
