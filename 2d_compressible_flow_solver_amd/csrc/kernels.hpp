@@ -1264,24 +1264,37 @@ __device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, d
     }
 }
 
-// The TVD-RK combination of one cell (TVD2rdRK.cpp:3-31, TVD3rdRK.cpp:3-48)
-// from its state u, right-hand side L and the RK snapshots, and the store of
-// the new state (immersed ghosts were written while staging the tile).  NT:
-// the snapshots are read and the state written with non-temporal accesses.
+// The RK snapshots of cell c the stage's form combines (u0 for forms 1 and 2,
+// u1 for form 2), read before the cell's flux divergence is formed so that
+// the loads are in flight meanwhile.  NT: non-temporal loads.
 template <bool NT>
-__device__ __forceinline__ void update_cell(const StageArgs& a, long long c, int ty, double dt, const double u[4],
-                                            const double L[4], double res[4]) {
-    const double c2dt = a.c2 * dt;
+__device__ __forceinline__ void load_snapshots(const StageArgs& a, long long c, double s0[4], double s1[4]) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const gptr<const double> p0 = G(a.u0) + q * a.plane + c, p1 = G(a.u1) + q * a.plane + c;
+        s0[q] = 0.0;
+        s1[q] = 0.0;
+        if (a.form == 1 || a.form == 2) s0[q] = NT ? __builtin_nontemporal_load(p0) : *p0;
+        if (a.form == 2) s1[q] = NT ? __builtin_nontemporal_load(p1) : *p1;
+    }
+}
+
+// The TVD-RK combination of one cell (TVD2rdRK.cpp:3-31, TVD3rdRK.cpp:3-48)
+// from its state u, right-hand side L and the RK snapshots s0, s1, and the
+// store of the new state (immersed ghosts were written while staging the
+// tile).  NT: the state is written with non-temporal stores.
+template <bool NT>
+__device__ __forceinline__ void update_cell(const StageArgs& a, long long c, int ty, double dt, const double u[4],
+                                            const double L[4], const double s0[4], const double s1[4],
+                                            double res[4]) {
+    const double c2dt = a.c2 * dt;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
         res[q] = u[q];
         if (ty == 0) {
             if (a.form == 0) res[q] = a.c1 * u[q] + c2dt * L[q];
-            else if (a.form == 1) res[q] = (a.c0 * (NT ? __builtin_nontemporal_load(p0) : *p0) + a.c1 * u[q]) + c2dt * L[q];
-            else if (a.form == 2)
-                res[q] = (a.c0 * (NT ? __builtin_nontemporal_load(p0) : *p0) +
-                          a.c1 * (NT ? __builtin_nontemporal_load(p1) : *p1)) + c2dt * L[q];
+            else if (a.form == 1) res[q] = (a.c0 * s0[q] + a.c1 * u[q]) + c2dt * L[q];
+            else if (a.form == 2) res[q] = (a.c0 * s0[q] + a.c1 * s1[q]) + c2dt * L[q];
             else res[q] = L[q];
         } else if (a.form == 3) {
             res[q] = 0.0;
@@ -1487,6 +1500,17 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
             const long long c = (long long)i * a.ny + j;
             const int ty = (a.type == nullptr) ? 0 : G(a.type)[c];
             const R rdy = (R)a.dy, idy = (R)a.inv_dy;
+            // the two forms of the snapshot reads and state writes stay apart
+            // (the empty asm statements keep the compiler from merging them,
+            // which would drop the non-temporal hint)
+            double s0[4], s1[4];
+            if (a.stream_nt) {
+                asm volatile("" ::: "memory");
+                load_snapshots<true>(a, c, s0, s1);
+                asm volatile("" ::: "memory");
+            } else {
+                load_snapshots<false>(a, c, s0, s1);
+            }
             double u[4], L[4], res[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -1502,15 +1526,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
                     L[q] = (double)(txq - tyy);
                 }
             }
-            // the two forms of the snapshot reads and state writes stay apart
-            // (the empty asm statements keep the compiler from merging them,
-            // which would drop the non-temporal hint)
             if (a.stream_nt) {
                 asm volatile("" ::: "memory");
-                update_cell<true>(a, c, ty, dt, u, L, res);
+                update_cell<true>(a, c, ty, dt, u, L, s0, s1, res);
                 asm volatile("" ::: "memory");
             } else {
-                update_cell<false>(a, c, ty, dt, u, L, res);
+                update_cell<false>(a, c, ty, dt, u, L, s0, s1, res);
             }
             if (a.fuse_cfl && ty == 0)
                 smax = (sizeof(R) == 8)
