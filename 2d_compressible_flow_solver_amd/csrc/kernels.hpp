@@ -394,7 +394,7 @@ __device__ __forceinline__ void riemann_flux(const R L[4], const R Rr[4], const 
 
 template <typename R, bool FAST>
 __device__ __forceinline__ void face_flux(const R* __restrict__ S, int ns, const int cell[6], int nrm, int tng,
-                                          R g, int positivity, int hll_bounds, bool xg, R out[4]) {
+                                          R g, int positivity, int hll_bounds, bool xg, unsigned slow, R out[4]) {
     R L[4], Rr[4], cl[4], cr[4];
     const int comp[4] = {0, nrm, tng, 3};
 #pragma unroll
@@ -402,8 +402,13 @@ __device__ __forceinline__ void face_flux(const R* __restrict__ S, int ns, const
         R v[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k) v[k] = S[comp[q] * ns + cell[k]];
-        L[q] = weno5<R, FAST>(v[0], v[1], v[2], v[3], v[4]);
-        Rr[q] = weno5<R, FAST>(v[5], v[4], v[3], v[2], v[1]);
+        if (!FAST && ((slow >> comp[q]) & 1u)) {   // this variable's IEEE form (uniform per tile)
+            L[q] = weno5<R, false>(v[0], v[1], v[2], v[3], v[4]);
+            Rr[q] = weno5<R, false>(v[5], v[4], v[3], v[2], v[1]);
+        } else {
+            L[q] = weno5<R, true>(v[0], v[1], v[2], v[3], v[4]);
+            Rr[q] = weno5<R, true>(v[5], v[4], v[3], v[2], v[1]);
+        }
         cl[q] = v[2];
         cr[q] = v[3];
     }
@@ -1056,8 +1061,9 @@ __device__ __forceinline__ void viscous_flux(const ViscLDS<R, Sh>& V, const int 
 // (0 or n of a non-periodic axis), so every stencil is the plain 6 cells and
 // its LDS offsets are constants.
 template <typename R, bool VISC, bool LIMIT, bool FAST, typename Sh, int DIR, bool HALF, bool REG = false>
-__device__ __forceinline__ void tile_face(const StageArgs& a, int s, int i0, int j0, R g, bool xg, R lam_x, R lam_y,
-                                          const R* S, R* F, const ViscLDS<R, Sh>* V, const LimitLDS<R, Sh>* Lm) {
+__device__ __forceinline__ void tile_face(const StageArgs& a, int s, int i0, int j0, R g, bool xg, unsigned slow,
+                                          R lam_x, R lam_y, const R* S, R* F, const ViscLDS<R, Sh>* V,
+                                          const LimitLDS<R, Sh>* Lm) {
     constexpr int TJ = Sh::TJ, SC = Sh::SC, NXF = Sh::NXF;
     constexpr int NS = Sh::SR * Sh::SC, NF = Sh::template flux_plane<HALF>();   // LDS plane lengths
     const bool xf = DIR == 0 ? s < NXF : DIR == 1;
@@ -1092,7 +1098,7 @@ __device__ __forceinline__ void tile_face(const StageArgs& a, int s, int i0, int
         }
     }
     R fl[4];
-    face_flux<R, FAST>(S, NS, cell, nrm, tng, g, a.positivity, a.hll_bounds, xg, fl);
+    face_flux<R, FAST>(S, NS, cell, nrm, tng, g, a.positivity, a.hll_bounds, xg, slow, fl);
     if constexpr (LIMIT) {
         if (a.positivity == 2) {
             int pL, pR;
@@ -1125,8 +1131,8 @@ __device__ __forceinline__ void tile_face(const StageArgs& a, int s, int i0, int
 // The tile has one y-face fewer than x-faces: the last thread computes a copy
 // of the y-face before it and does not store it.
 template <typename R, bool LIMIT, bool FAST, typename Sh>
-__device__ __forceinline__ void tile_face_pair(const StageArgs& a, int s, int i0, int j0, R g, bool xg, R lam_x,
-                                               R lam_y, const R* S, R* F, const LimitLDS<R, Sh>* Lm) {
+__device__ __forceinline__ void tile_face_pair(const StageArgs& a, int s, int i0, int j0, R g, bool xg, unsigned slow,
+                                               R lam_x, R lam_y, const R* S, R* F, const LimitLDS<R, Sh>* Lm) {
     constexpr int TJ = Sh::TJ, SC = Sh::SC, NXF = Sh::NXF, NYF = Sh::NYF;
     constexpr int NS = Sh::SR * Sh::SC, NF = NXF + NYF;
     const int sy = s < NYF ? s : NYF - 1;
@@ -1165,8 +1171,13 @@ __device__ __forceinline__ void tile_face_pair(const StageArgs& a, int s, int i0
             R v[6];
 #pragma unroll
             for (int k = 0; k < 6; ++k) v[k] = S[comp[f][q] * NS + cell[f][k]];
-            L[f][q] = weno5<R, FAST>(v[0], v[1], v[2], v[3], v[4]);
-            Rr[f][q] = weno5<R, FAST>(v[5], v[4], v[3], v[2], v[1]);
+            if (!FAST && ((slow >> comp[f][q]) & 1u)) {
+                L[f][q] = weno5<R, false>(v[0], v[1], v[2], v[3], v[4]);
+                Rr[f][q] = weno5<R, false>(v[5], v[4], v[3], v[2], v[1]);
+            } else {
+                L[f][q] = weno5<R, true>(v[0], v[1], v[2], v[3], v[4]);
+                Rr[f][q] = weno5<R, true>(v[5], v[4], v[3], v[2], v[1]);
+            }
             cl[f][q] = v[2];
             cr[f][q] = v[3];
         }
@@ -1214,7 +1225,7 @@ __device__ __forceinline__ R rhs_x(const StageArgs& a, const R* F, int q, int il
 // each cell has taken the x part of its divergence into tx (the cells of
 // threads tid < TI * TJ).
 template <typename R, bool VISC, bool LIMIT, bool FAST, typename Sh, bool SPLIT, bool HALF>
-__device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, double dt, const R* S, R* F,
+__device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, double dt, unsigned slow, const R* S, R* F,
                                            const ViscLDS<R, Sh>* V, const LimitLDS<R, Sh>* Lm, R tx[4]) {
     static_assert(!HALF || SPLIT, "the halved flux planes need the x-faces first");
     constexpr int NXF = Sh::NXF, NF = Sh::NXF + Sh::NYF;
@@ -1229,7 +1240,7 @@ __device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, d
     const R lam_y = (R)(a.dy_pow2 ? dt * a.inv_dy : dt / a.dy);
     if constexpr (SPLIT && Sh::DUAL) {
         static_assert(!HALF && !VISC, "both directions' fluxes are live at once; inviscid only");
-        for (int s = tid; s < NXF; s += Sh::NT) tile_face_pair<R, LIMIT, FAST, Sh>(a, s, i0, j0, g, xg, lam_x, lam_y, S, F, Lm);
+        for (int s = tid; s < NXF; s += Sh::NT) tile_face_pair<R, LIMIT, FAST, Sh>(a, s, i0, j0, g, xg, slow, lam_x, lam_y, S, F, Lm);
     } else if constexpr (SPLIT) {
         // a direction whose tile faces include no degenerate boundary face
         // (the fast division forms only: the IEEE form serves rare tiles)
@@ -1237,10 +1248,10 @@ __device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, d
         const bool yreg = FAST && (a.bc[2] == BC_PERIODIC || (j0 > 0 && j0 + Sh::TJ < a.ny));
         if (xreg) {
             for (int s = tid; s < NXF; s += Sh::NT)
-                tile_face<R, VISC, LIMIT, FAST, Sh, 1, HALF, FAST>(a, s, i0, j0, g, xg, lam_x, lam_y, S, F, V, Lm);
+                tile_face<R, VISC, LIMIT, FAST, Sh, 1, HALF, FAST>(a, s, i0, j0, g, xg, slow, lam_x, lam_y, S, F, V, Lm);
         } else {
             for (int s = tid; s < NXF; s += Sh::NT)
-                tile_face<R, VISC, LIMIT, FAST, Sh, 1, HALF>(a, s, i0, j0, g, xg, lam_x, lam_y, S, F, V, Lm);
+                tile_face<R, VISC, LIMIT, FAST, Sh, 1, HALF>(a, s, i0, j0, g, xg, slow, lam_x, lam_y, S, F, V, Lm);
         }
         if constexpr (HALF) {
             __syncthreads();
@@ -1253,14 +1264,14 @@ __device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, d
         }
         if (yreg) {
             for (int s = NXF + tid; s < NF; s += Sh::NT)
-                tile_face<R, VISC, LIMIT, FAST, Sh, 2, HALF, FAST>(a, s, i0, j0, g, xg, lam_x, lam_y, S, F, V, Lm);
+                tile_face<R, VISC, LIMIT, FAST, Sh, 2, HALF, FAST>(a, s, i0, j0, g, xg, slow, lam_x, lam_y, S, F, V, Lm);
         } else {
             for (int s = NXF + tid; s < NF; s += Sh::NT)
-                tile_face<R, VISC, LIMIT, FAST, Sh, 2, HALF>(a, s, i0, j0, g, xg, lam_x, lam_y, S, F, V, Lm);
+                tile_face<R, VISC, LIMIT, FAST, Sh, 2, HALF>(a, s, i0, j0, g, xg, slow, lam_x, lam_y, S, F, V, Lm);
         }
     } else {
         for (int s = tid; s < NF; s += Sh::NT)
-            tile_face<R, VISC, LIMIT, FAST, Sh, 0, HALF>(a, s, i0, j0, g, xg, lam_x, lam_y, S, F, V, Lm);
+            tile_face<R, VISC, LIMIT, FAST, Sh, 0, HALF>(a, s, i0, j0, g, xg, slow, lam_x, lam_y, S, F, V, Lm);
     }
 }
 
@@ -1310,7 +1321,10 @@ __device__ __forceinline__ void update_cell(const StageArgs& a, long long c, int
     }
 }
 
-template <typename R, bool VISC, bool LIMIT, typename Sh, bool SPLIT = true, bool HALF = false>
+// PER_VAR: a tile outside the fast WENO5 range takes the IEEE form for the
+// variables that hold the out-of-range values only (else for all four: the
+// batched kernel, whose registers do not take both forms of every variable).
+template <typename R, bool VISC, bool LIMIT, typename Sh, bool SPLIT = true, bool HALF = false, bool PER_VAR = true>
 __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R* F, double* wmax,
                                            ViscLDS<R, Sh>* V, LimitLDS<R, Sh>* Lm) {
     constexpr int TI = Sh::TI, TJ = Sh::TJ, SR = Sh::SR, SC = Sh::SC, NXF = Sh::NXF, BLOCK = Sh::NT;
@@ -1364,7 +1378,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
     } else {
         dt = *a.dt;
     }
-    bool unsafe = false;
+    bool unsafe_q[4] = {false, false, false, false};   // per variable (state plane)
 
     // ---- tile + halo -> LDS (corners are never read) ----
     const bool near_ib = a.tile_ib != nullptr && G(a.tile_ib)[tile];
@@ -1407,7 +1421,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             S[q * NS + s] = (R)v[q];
-            if (sizeof(R) == 8) unsafe = unsafe || !fast_div_safe(v[q]);
+            if (sizeof(R) == 8) unsafe_q[q] = unsafe_q[q] || !fast_div_safe(v[q]);
         }
         if constexpr (LIMIT) {
             if (limit)
@@ -1426,8 +1440,23 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
     }
     // one barrier + OR over the tile: tiles outside the range of the fast
     // WENO5 division forms take the IEEE form (and are counted)
+    const bool unsafe = unsafe_q[0] || unsafe_q[1] || unsafe_q[2] || unsafe_q[3];
     const bool tile_unsafe = __syncthreads_or(unsafe ? 1 : 0) != 0;
-    if (tile_unsafe && tid == 0) atomicAdd(a.exact_count, 1);
+    // such a tile takes the IEEE form for the variables that hold the
+    // out-of-range values only (bit p: state plane p), OR-ed over the block
+    unsigned slow = PER_VAR ? 0u : 0xFu;
+    if (sizeof(R) == 8 && PER_VAR && tile_unsafe) {
+        unsigned wm = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) wm |= (__ballot(unsafe_q[q] ? 1 : 0) != 0ull) ? (1u << q) : 0u;
+        unsigned* m = reinterpret_cast<unsigned*>(wmax);   // free until the block maximum at the end
+        if ((tid & 63) == 0) m[tid >> 6] = wm;
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < BLOCK / 64; ++w) slow |= m[w];
+        __syncthreads();
+    }
+    if (sizeof(R) == 8 && tile_unsafe && tid == 0) atomicAdd(a.exact_count, 1);
 
     if constexpr (LIMIT) {
         if (a.positivity == 2) {
@@ -1485,9 +1514,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
     // ---- faces: the exact-division form on tiles outside the fast range ----
     R tx[4];   // with HALF: the x part of the cell's flux divergence
     if (sizeof(R) == 8 && tile_unsafe)
-        tile_faces<R, VISC, LIMIT, false, Sh, SPLIT, HALF>(a, i0, j0, dt, S, F, V, Lm, tx);
+        tile_faces<R, VISC, LIMIT, false, Sh, SPLIT, HALF>(a, i0, j0, dt, slow, S, F, V, Lm, tx);
     else
-        tile_faces<R, VISC, LIMIT, true, Sh, SPLIT, HALF>(a, i0, j0, dt, S, F, V, Lm, tx);
+        tile_faces<R, VISC, LIMIT, true, Sh, SPLIT, HALF>(a, i0, j0, dt, 0u, S, F, V, Lm, tx);
     __syncthreads();
 
     // ---- cell update: flux divergence (Cell::comRHS, Cell.cpp:110-125) and the
@@ -1627,11 +1656,11 @@ __global__ __launch_bounds__(Sh::NT, (Occupancy<R, false, LIMIT, Sh>::batch_wave
         const int2 pt = tiles[t];
         constexpr bool split = Occupancy<R, false, LIMIT, Sh>::batch_split;
         if constexpr (LIMIT)
-            stage_tile<R, false, true, Sh, split>(descs[(pt.x * 2 + parity) * 3 + sub], pt.y, S, F, wmax, nullptr,
-                                                  &Lm);
+            stage_tile<R, false, true, Sh, split, false, false>(descs[(pt.x * 2 + parity) * 3 + sub], pt.y, S, F,
+                                                                wmax, nullptr, &Lm);
         else
-            stage_tile<R, false, false, Sh, split>(descs[(pt.x * 2 + parity) * 3 + sub], pt.y, S, F, wmax, nullptr,
-                                                   nullptr);
+            stage_tile<R, false, false, Sh, split, false, false>(descs[(pt.x * 2 + parity) * 3 + sub], pt.y, S, F,
+                                                                 wmax, nullptr, nullptr);
         __syncthreads();   // the next tile reuses S, F and wmax
     }
 }
