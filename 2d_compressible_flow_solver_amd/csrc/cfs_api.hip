@@ -113,10 +113,14 @@ dim3 stage_grid(const cfs_solver* s) { return dim3((unsigned)(8 * ((s->dev.tiles
 
 // The tile shape a new handle starts with: the dual-face shape 3 for
 // inviscid grids of at most 5000 tiles of 15 x 16 (up to about 1100^2
-// cells), where its two chains per thread beat shape 0 by 1-4% (Riemann
-// problems 256^2 to 1024^2, Lax-Liu 400^2, both precisions); shape 0 above
-// (the 2048 x 1024 cylinder: 1% the other way).  cfs_autotune measures instead.
-int default_shape(const cfs_config& c) { return (!c.viscous && tiles_for(c, 0) <= 5000) ? 3 : 0; }
+// cells) without the positivity limiter, where its two chains per thread
+// beat shape 0 (Lax-Liu 400^2: 7%); shape 0 above (the 2048 x 1024 cylinder:
+// 1% the other way) and with the limiter (served Riemann problems 512^2 to
+// 1024^2: shape 3 is 0-1% slower in FP64, up to 7% in FP32-bulk).
+// cfs_autotune measures instead.
+int default_shape(const cfs_config& c) {
+    return (!c.viscous && c.positivity != 2 && tiles_for(c, 0) <= 5000) ? 3 : 0;
+}
 
 // Makes tile shape k current: tile counts and the IB tile flags of that shape.
 void apply_shape(cfs_solver* s, int k) {

@@ -1324,7 +1324,10 @@ __device__ __forceinline__ void update_cell(const StageArgs& a, long long c, int
 // PER_VAR: a tile outside the fast WENO5 range takes the IEEE form for the
 // variables that hold the out-of-range values only (else for all four: the
 // batched kernel, whose registers do not take both forms of every variable).
-template <typename R, bool VISC, bool LIMIT, typename Sh, bool SPLIT = true, bool HALF = false, bool PER_VAR = true>
+// EARLY: the first staging pass is read before the time step is derived (the
+// batched kernel, which also holds its problem's arguments, spills with it).
+template <typename R, bool VISC, bool LIMIT, typename Sh, bool SPLIT = true, bool HALF = false, bool PER_VAR = true,
+          bool EARLY = true>
 __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R* F, double* wmax,
                                            ViscLDS<R, Sh>* V, LimitLDS<R, Sh>* Lm) {
     constexpr int TI = Sh::TI, TJ = Sh::TJ, SR = Sh::SR, SC = Sh::SC, NXF = Sh::NXF, BLOCK = Sh::NT;
@@ -1338,46 +1341,6 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
 
     if (tile == 0 && tid == 0) *a.exact_other = 0;   // the previous stage's counter, consumed
 
-    // ---- time step ----
-    double dt;
-    if (a.dt_mode == 1) {
-        // first stage of a CFL step: every workgroup derives the same dt from
-        // the bound accumulated by the previous step's last stage
-        const double t = *a.t_in;
-        // a halt is sticky until the host clears it (workgroup 0 rewrites the
-        // flag with the value every workgroup derives, so a read racing with
-        // that write sees the same decision)
-        bool stop = *a.halt != 0 || (a.t_end > 0.0 && !(t < a.t_end));
-        bool bad = false;
-        double d = 0.0;
-        if (!stop) {
-            const double s = __longlong_as_double((long long)*a.acc_in);
-            d = a.cfl / s;
-            if (!(s > 0.0) || !(d < 1.0e300)) {
-                bad = true;
-                stop = true;
-                d = 0.0;
-            } else if (a.t_end > 0.0) {
-                const double rem = a.t_end - t;
-                if (!(d < rem)) d = rem > 0.0 ? rem : 0.0;
-            }
-        }
-        if (tile == 0 && tid == 0) {
-            *a.dt = d;
-            *a.t_out = t + d;
-            *a.halt = stop ? 1 : 0;
-            if (bad) *a.status = 1;
-            if (!stop) *a.steps += 1;
-            *a.acc_out = 0ull;   // consumed two steps ago; this step's last stage fills it
-        }
-        if (stop) return;
-        dt = d;
-    } else if (a.dt_mode == 2) {
-        if (*a.halt) return;
-        dt = *a.dt;
-    } else {
-        dt = *a.dt;
-    }
     bool unsafe_q[4] = {false, false, false, false};   // per variable (state plane)
 
     // ---- tile + halo -> LDS (corners are never read) ----
@@ -1428,7 +1391,61 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
                 limit_cell<R, Sh>(s, r - HALO, c - HALO, (R)v[0], (R)v[1], (R)v[2], (R)v[3], (R)a.gamma, F, Lm);
         }
     };
-    for (int s0 = tid; s0 < SR * SC; s0 += PER * BLOCK) {
+    // the first pass's loads are issued before the time step is known: the
+    // step's scalars and the tile's cells are read in one memory latency
+    // (nothing is written before the time step decides that the stage runs;
+    // the first pass is peeled off the loop, which also keeps the FP32-bulk
+    // kernels free of register spills)
+    double f0[4], f1[4];
+    long long fc0 = -1, fc1 = -1;
+    const bool fn0 = EARLY && needed(tid), fn1 = EARLY && PER == 2 && tid + BLOCK < SR * SC && needed(tid + BLOCK);
+    if (fn0) fc0 = load(tid, f0);
+    if (fn1) fc1 = load(tid + BLOCK, f1);
+
+    // ---- time step ----
+    double dt;
+    if (a.dt_mode == 1) {
+        // first stage of a CFL step: every workgroup derives the same dt from
+        // the bound accumulated by the previous step's last stage
+        const double t = *a.t_in;
+        // a halt is sticky until the host clears it (workgroup 0 rewrites the
+        // flag with the value every workgroup derives, so a read racing with
+        // that write sees the same decision)
+        bool stop = *a.halt != 0 || (a.t_end > 0.0 && !(t < a.t_end));
+        bool bad = false;
+        double d = 0.0;
+        if (!stop) {
+            const double s = __longlong_as_double((long long)*a.acc_in);
+            d = a.cfl / s;
+            if (!(s > 0.0) || !(d < 1.0e300)) {
+                bad = true;
+                stop = true;
+                d = 0.0;
+            } else if (a.t_end > 0.0) {
+                const double rem = a.t_end - t;
+                if (!(d < rem)) d = rem > 0.0 ? rem : 0.0;
+            }
+        }
+        if (tile == 0 && tid == 0) {
+            *a.dt = d;
+            *a.t_out = t + d;
+            *a.halt = stop ? 1 : 0;
+            if (bad) *a.status = 1;
+            if (!stop) *a.steps += 1;
+            *a.acc_out = 0ull;   // consumed two steps ago; this step's last stage fills it
+        }
+        if (stop) return;
+        dt = d;
+    } else if (a.dt_mode == 2) {
+        if (*a.halt) return;
+        dt = *a.dt;
+    } else {
+        dt = *a.dt;
+    }
+
+    if (fn0) store(tid, f0, fc0);
+    if (fn1) store(tid + BLOCK, f1, fc1);
+    for (int s0 = tid + (EARLY ? PER * BLOCK : 0); s0 < SR * SC; s0 += PER * BLOCK) {
         const int s1 = s0 + BLOCK;
         const bool n0 = needed(s0), n1 = PER == 2 && s1 < SR * SC && needed(s1);
         double v0[4], v1[4];
@@ -1656,11 +1673,11 @@ __global__ __launch_bounds__(Sh::NT, (Occupancy<R, false, LIMIT, Sh>::batch_wave
         const int2 pt = tiles[t];
         constexpr bool split = Occupancy<R, false, LIMIT, Sh>::batch_split;
         if constexpr (LIMIT)
-            stage_tile<R, false, true, Sh, split, false, false>(descs[(pt.x * 2 + parity) * 3 + sub], pt.y, S, F,
-                                                                wmax, nullptr, &Lm);
+            stage_tile<R, false, true, Sh, split, false, false, false>(descs[(pt.x * 2 + parity) * 3 + sub], pt.y,
+                                                                       S, F, wmax, nullptr, &Lm);
         else
-            stage_tile<R, false, false, Sh, split, false, false>(descs[(pt.x * 2 + parity) * 3 + sub], pt.y, S, F,
-                                                                 wmax, nullptr, nullptr);
+            stage_tile<R, false, false, Sh, split, false, false, false>(descs[(pt.x * 2 + parity) * 3 + sub], pt.y,
+                                                                        S, F, wmax, nullptr, nullptr);
         __syncthreads();   // the next tile reuses S, F and wmax
     }
 }

@@ -264,8 +264,9 @@ int32_t cfs_last_kernel_ms(const cfs_solver* s, double* ms_per_stage);
  * (15 x 16, 256 threads) is the only one of the viscous variant; shape 3 has
  * shape 0's tile with each thread computing its x-face and its y-face in one
  * pass (two independent chains per thread).  A new handle starts with shape
- * 3 for inviscid grids of at most 5000 shape-0 tiles and with shape 0
- * otherwise.  No reference counterpart: the reference has no tiling. */
+ * 3 for inviscid grids of at most 5000 shape-0 tiles without the positivity
+ * limiter and with shape 0 otherwise.  No reference counterpart: the
+ * reference has no tiling. */
 #define CFS_NUM_TILE_SHAPES 4
 int32_t cfs_tile_shape_info(int32_t shape, int32_t* ti, int32_t* tj, int32_t* threads);
 /* Selects the tile shape of later stages (CFS_E_UNSUPPORTED for a viscous
