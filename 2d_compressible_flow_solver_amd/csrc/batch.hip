@@ -87,6 +87,7 @@ struct cfs_batch {
     StageArgs* descs = nullptr;       // [max_problems][2][3]
     StageArgs* descs_host = nullptr;  // pinned staging for one slot
     int2* tiles = nullptr;            // (slot, tile) list
+    int* sched = nullptr;             // tile counters of the batched kernel (BATCH_SCHED_INTS)
     int* ntiles = nullptr;
     long long tiles_cap = 0;
     int2* tiles_host = nullptr;       // pinned staging of the list
@@ -107,7 +108,7 @@ namespace {
 template <typename R, bool LIMIT, typename Sh>
 int launch_batch_shape(cfs_batch* b, int parity, int sub) {
     hipLaunchKernelGGL((batch_stage_kernel<R, LIMIT, Sh>), dim3((unsigned)b->grid), dim3(Sh::NT), 0, b->stream,
-                       b->descs, b->tiles, b->ntiles, parity, sub);
+                       b->descs, b->tiles, b->ntiles, parity, sub, b->sched);
     HIP_OK(hipGetLastError());
     return CFS_OK;
 }
@@ -164,6 +165,7 @@ int rebuild_tiles(cfs_batch* b) {
 }
 
 void drop_graph(cfs_batch* b) {
+    SetupLock lk(hip_setup_mutex());
     if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->graph8_exec) (void)hipGraphExecDestroy(b->graph8_exec);
@@ -194,6 +196,7 @@ int32_t cfs_batch_create(const cfs_batch_config* bc, void* hip_stream, cfs_batch
         return CFS_E_ARG;
     if (bc->limiter != 0 && bc->limiter != 1) return CFS_E_ARG;
     if (bc->tile_shape < 0 || bc->tile_shape >= NUM_SHAPES) return CFS_E_ARG;
+    SetupLock lk(hip_setup_mutex());
     cfs_batch* b = new (std::nothrow) cfs_batch();
     if (!b) return CFS_E_OOM;
     b->bc = *bc;
@@ -215,12 +218,12 @@ int32_t cfs_batch_create(const cfs_batch_config* bc, void* hip_stream, cfs_batch
                  : (lim ? blocks_per_cu_shape<float, true>(k, &per_cu) : blocks_per_cu_shape<float, false>(k, &per_cu));
     }
     if (e != hipSuccess) { cfs_batch_destroy(b); return CFS_E_HIP; }
-    // four times the workgroups the CUs hold at once: each takes a quarter of
-    // the tiles, and the hardware starts the waiting ones as others finish, so
-    // tiles of uneven cost (body, limiter, halted problems) even out
+    // the workgroups the CUs hold at once: they take the tiles in list order
+    // from a counter per XCD range, so tiles of uneven cost (body, limiter,
+    // halted problems) even out without more workgroups than fit
     b->device = dev;
     b->resident = cus * (per_cu > 0 ? per_cu : 1);
-    int g = bc->workgroups > 0 ? bc->workgroups : 4 * b->resident;
+    int g = bc->workgroups > 0 ? bc->workgroups : b->resident;
     b->grid = ((g + 7) / 8) * 8;
     // tile list capacity: twice the full tiles an arena of problem planes
     // holds (96 B per cell) plus 16 ragged tiles per slot; admission past it
@@ -234,6 +237,7 @@ int32_t cfs_batch_create(const cfs_batch_config* bc, void* hip_stream, cfs_batch
         hipMalloc(&b->descs, sizeof(StageArgs) * DESCS_PER_SLOT * np) != hipSuccess ||
         hipMalloc(&b->tiles, sizeof(int2) * (size_t)b->tiles_cap) != hipSuccess ||
         hipMalloc(&b->ntiles, sizeof(int)) != hipSuccess ||
+        hipMalloc(&b->sched, sizeof(int) * BATCH_SCHED_INTS) != hipSuccess ||
         hipHostMalloc(&b->descs_host, sizeof(StageArgs) * DESCS_PER_SLOT, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc(&b->tiles_host, sizeof(int2) * (size_t)b->tiles_cap, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc(&b->scal_host, sizeof(double) * SCAL_DOUBLES * np, hipHostMallocDefault) != hipSuccess) {
@@ -244,6 +248,7 @@ int32_t cfs_batch_create(const cfs_batch_config* bc, void* hip_stream, cfs_batch
     if (hipMemsetAsync(b->scal, 0, sizeof(double) * SCAL_DOUBLES * np, b->stream) != hipSuccess ||
         hipMemsetAsync(b->exact, 0, sizeof(int) * 2 * np, b->stream) != hipSuccess ||
         hipMemsetAsync(b->ntiles, 0, sizeof(int), b->stream) != hipSuccess ||
+        hipMemsetAsync(b->sched, 0, sizeof(int) * BATCH_SCHED_INTS, b->stream) != hipSuccess ||
         hipStreamSynchronize(b->stream) != hipSuccess) {
         cfs_batch_destroy(b);
         return CFS_E_HIP;
@@ -255,6 +260,7 @@ int32_t cfs_batch_create(const cfs_batch_config* bc, void* hip_stream, cfs_batch
 void cfs_batch_destroy(cfs_batch* b) {
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    SetupLock lk(hip_setup_mutex());
     drop_graph(b);
     (void)hipFree(b->arena);
     (void)hipFree(b->scal);
@@ -262,6 +268,7 @@ void cfs_batch_destroy(cfs_batch* b) {
     (void)hipFree(b->descs);
     (void)hipFree(b->tiles);
     (void)hipFree(b->ntiles);
+    (void)hipFree(b->sched);
     (void)hipHostFree(b->descs_host);
     (void)hipHostFree(b->tiles_host);
     (void)hipHostFree(b->scal_host);
@@ -411,6 +418,7 @@ int32_t cfs_batch_admit(cfs_batch* b, const cfs_config* cfg, const double* U, co
 // A graph of `steps` RK steps (even: the step parities alternate), starting at
 // parity 0.
 static int capture_steps(cfs_batch* b, int steps, hipGraph_t* graph, hipGraphExec_t* exec) {
+    SetupLock lk(hip_setup_mutex());
     HIP_OK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     int crc = 0;
     for (int k = 0; k < steps && !crc; ++k) crc = launch_batch_step(b, k & 1);
@@ -514,15 +522,21 @@ int32_t cfs_batch_autotune(const cfs_batch_config* bc, const cfs_request* sample
         }
         hipEvent_t e0 = nullptr, e1 = nullptr;
         float ms = 0.f;
-        if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = CFS_E_HIP;
+        {
+            SetupLock lk(hip_setup_mutex());
+            if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = CFS_E_HIP;
+        }
         if (!rc) rc = cfs_batch_step(b, 2);
         if (!rc && hipEventRecord(e0, b->stream) != hipSuccess) rc = CFS_E_HIP;
         if (!rc) rc = cfs_batch_step(b, steps);
         if (!rc && (hipEventRecord(e1, b->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                     hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
             rc = CFS_E_HIP;
-        if (e0) (void)hipEventDestroy(e0);
-        if (e1) (void)hipEventDestroy(e1);
+        {
+            SetupLock lk(hip_setup_mutex());
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+        }
         cfs_batch_destroy(b);
         if (rc) return rc;
         const double t = (double)ms / steps;

@@ -74,6 +74,7 @@ namespace {
 ScalPtrs scal(cfs_solver* s) { return s->dev.s(); }
 
 void drop_graph(cfs_solver* s) {
+    SetupLock lk(hip_setup_mutex());
     if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
     if (s->graph) (void)hipGraphDestroy(s->graph);
     s->graph_exec = nullptr;
@@ -138,6 +139,7 @@ int launch_stage(cfs_solver* s, int counter, bool timed, const StageTime& tm) {
     const int next = fill_stage_args(a, s->dev, rk_substage(c.rk, counter), s->step_parity, s->parity, tm);
     const dim3 grid = stage_grid(s);
     if (timed) {
+        SetupLock lk(hip_setup_mutex());
         hipEvent_t e0, e1;
         HIP_OK(hipEventCreate(&e0));
         HIP_OK(hipEventCreate(&e1));
@@ -189,6 +191,7 @@ int set_dt_device(cfs_solver* s, double dt) {
 }
 
 void free_ib(cfs_solver* s) {
+    SetupLock lk(hip_setup_mutex());
     ProblemDev& d = s->dev;
     (void)hipFree(d.ig_cells);
     (void)hipFree(d.ig_donors);
@@ -240,6 +243,7 @@ int32_t cfs_create(const cfs_config* cfg, void* hip_stream, cfs_solver** out) {
     *out = nullptr;
     int rc = check_config(cfg);
     if (rc) return rc;
+    SetupLock lk(hip_setup_mutex());
     cfs_solver* s = new (std::nothrow) cfs_solver();
     if (!s) return CFS_E_OOM;
     s->cfg = *cfg;
@@ -271,6 +275,7 @@ int32_t cfs_create(const cfs_config* cfg, void* hip_stream, cfs_solver** out) {
 void cfs_destroy(cfs_solver* s) {
     if (!s) return;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    SetupLock lk(hip_setup_mutex());
     drop_graph(s);
     for (auto e : s->ev) (void)hipEventDestroy(e);
     free_ib(s);
@@ -334,10 +339,13 @@ int32_t cfs_set_immersed(cfs_solver* s, int32_t interp, const uint8_t* cell_type
     if (rc) return rc;
     for (int k = 1; k < NUM_SHAPES; ++k) ib_tile_flags(s->cfg, k, n_ig, ig_cells, tiles[k]);
     HIP_OK(hipStreamSynchronize(s->stream));
+    SetupLock lk(hip_setup_mutex());
     drop_graph(s);
     free_ib(s);
     if (!d.type && hipMalloc(&d.type, (size_t)d.plane) != hipSuccess) return CFS_E_OOM;
-    HIP_OK(hipMemcpy(d.type, cell_type, (size_t)d.plane, hipMemcpyHostToDevice));
+    // copies on the handle's stream (the legacy default stream would meet
+    // other threads' streams), waited for before the host tables go away
+    HIP_OK(hipMemcpyAsync(d.type, cell_type, (size_t)d.plane, hipMemcpyHostToDevice, s->stream));
     if (n_ig > 0) {
         if (hipMalloc(&d.ig_cells, sizeof(int) * n_ig) != hipSuccess ||
             hipMalloc(&d.ig_donors, sizeof(int) * 4 * n_ig) != hipSuccess ||
@@ -353,15 +361,18 @@ int32_t cfs_set_immersed(cfs_solver* s, int32_t interp, const uint8_t* cell_type
                 free_ib(s);
                 return CFS_E_OOM;
             }
-        HIP_OK(hipMemcpy(d.ig_cells, ig_cells, sizeof(int) * n_ig, hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy(d.ig_donors, donors, sizeof(int) * 4 * n_ig, hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy(d.ig_xy, image_xy, sizeof(double) * 2 * n_ig, hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy(d.ig_ldl, ldlt, sizeof(double) * 10 * n_ig, hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy(d.ig_transp, transp, sizeof(int) * 4 * n_ig, hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy(d.ig_index, index.data(), sizeof(int) * (size_t)d.plane, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpyAsync(d.ig_cells, ig_cells, sizeof(int) * n_ig, hipMemcpyHostToDevice, s->stream));
+        HIP_OK(hipMemcpyAsync(d.ig_donors, donors, sizeof(int) * 4 * n_ig, hipMemcpyHostToDevice, s->stream));
+        HIP_OK(hipMemcpyAsync(d.ig_xy, image_xy, sizeof(double) * 2 * n_ig, hipMemcpyHostToDevice, s->stream));
+        HIP_OK(hipMemcpyAsync(d.ig_ldl, ldlt, sizeof(double) * 10 * n_ig, hipMemcpyHostToDevice, s->stream));
+        HIP_OK(hipMemcpyAsync(d.ig_transp, transp, sizeof(int) * 4 * n_ig, hipMemcpyHostToDevice, s->stream));
+        HIP_OK(hipMemcpyAsync(d.ig_index, index.data(), sizeof(int) * (size_t)d.plane, hipMemcpyHostToDevice,
+                              s->stream));
         for (int k = 0; k < NUM_SHAPES; ++k)
-            HIP_OK(hipMemcpy(s->tile_ib_shape[k], tiles[k].data(), tiles[k].size(), hipMemcpyHostToDevice));
+            HIP_OK(hipMemcpyAsync(s->tile_ib_shape[k], tiles[k].data(), tiles[k].size(), hipMemcpyHostToDevice,
+                                  s->stream));
     }
+    HIP_OK(hipStreamSynchronize(s->stream));
     d.n_ig = n_ig;
     d.ib_interp = interp;
     apply_shape(s, s->shape);
@@ -400,8 +411,11 @@ int32_t cfs_advance(cfs_solver* s, int32_t nsteps, double dt, double cfl, double
     const bool use_cfl = !(dt > 0.0);
     if (use_cfl && !(cfl > 0.0)) return CFS_E_ARG;
     const double t0 = t_io ? *t_io : 0.0;
-    for (auto e : s->ev) (void)hipEventDestroy(e);
-    s->ev.clear();
+    {
+        SetupLock lk(hip_setup_mutex());
+        for (auto e : s->ev) (void)hipEventDestroy(e);
+        s->ev.clear();
+    }
     const bool timed = s->kernel_timing;
     int rc;
     int done = 0;
@@ -457,6 +471,7 @@ int32_t cfs_advance(cfs_solver* s, int32_t nsteps, double dt, double cfl, double
                               s->graph_key.stage_parity == key.stage_parity && s->graph_key.steps == key.steps;
             const int counter0 = s->counter;
             if (!same) {
+                SetupLock lk(hip_setup_mutex());
                 drop_graph(s);
                 HIP_OK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
                 int crc = 0;
@@ -498,11 +513,11 @@ int32_t cfs_advance(cfs_solver* s, int32_t nsteps, double dt, double cfl, double
             rc = launch_step(s, timed, true, cfl, t_end);
             if (rc) return rc;
         }
-        HIP_OK(hipStreamSynchronize(s->stream));
         int ints[2] = {0, 0};
-        HIP_OK(hipMemcpy(ints, scal(s).halt(), sizeof(ints), hipMemcpyDeviceToHost));
-        HIP_OK(hipMemcpy(&done, scal(s).steps(), sizeof(int), hipMemcpyDeviceToHost));
-        HIP_OK(hipMemcpy(&t, scal(s).t(s->step_parity), sizeof(double), hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpyAsync(ints, scal(s).halt(), sizeof(ints), hipMemcpyDeviceToHost, s->stream));
+        HIP_OK(hipMemcpyAsync(&done, scal(s).steps(), sizeof(int), hipMemcpyDeviceToHost, s->stream));
+        HIP_OK(hipMemcpyAsync(&t, scal(s).t(s->step_parity), sizeof(double), hipMemcpyDeviceToHost, s->stream));
+        HIP_OK(hipStreamSynchronize(s->stream));
         if (ints[1]) {
             if (t_io) *t_io = t;
             if (steps_done) *steps_done = done;
@@ -657,9 +672,12 @@ int32_t cfs_last_rhs(cfs_solver* s, double* R) {
     if (s->counter <= 1) return CFS_E_STATE;
     const cfs_config& c = s->cfg;
     const size_t bytes = sizeof(double) * 4 * (size_t)s->dev.plane;
-    if (!s->rhs && hipMalloc(&s->rhs, bytes + 2 * sizeof(int)) != hipSuccess) {
-        s->rhs = nullptr;
-        return CFS_E_OOM;
+    {
+        SetupLock lk(hip_setup_mutex());
+        if (!s->rhs && hipMalloc(&s->rhs, bytes + 2 * sizeof(int)) != hipSuccess) {
+            s->rhs = nullptr;
+            return CFS_E_OOM;
+        }
     }
     // the last stage's input plane-set is intact after it (every stage writes
     // another set): evaluate its RHS again, with its immersed-ghost fill
@@ -710,12 +728,15 @@ int32_t cfs_autotune(cfs_solver* s, int32_t reps, int32_t* best, double* ms) {
     const int nshape = s->cfg.viscous ? 1 : NUM_SHAPES;
     const int shape0 = s->shape;
     int* cnt = nullptr;
-    if (hipMalloc(&cnt, 2 * sizeof(int)) != hipSuccess) return CFS_E_OOM;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = CFS_OK;
     double t[NUM_SHAPES];
     int k_best = 0;
-    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = CFS_E_HIP;
+    {
+        SetupLock lk(hip_setup_mutex());
+        if (hipMalloc(&cnt, 2 * sizeof(int)) != hipSuccess) return CFS_E_OOM;
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = CFS_E_HIP;
+    }
     if (!rc) rc = set_dt_device(s, 0.0);
     // sub-stage 0 of the current state, fixed dt = 0, into plane-set 1: the
     // stage kernel's whole work (tile load with the boundary and IB fills,
@@ -747,10 +768,13 @@ int32_t cfs_autotune(cfs_solver* s, int32_t reps, int32_t* best, double* ms) {
         if (ms) ms[k] = t[k];
         if (k == 0 || t[k] < t[k_best]) k_best = k;
     }
-    if (e0) (void)hipEventDestroy(e0);
-    if (e1) (void)hipEventDestroy(e1);
     (void)hipStreamSynchronize(s->stream);
-    (void)hipFree(cnt);
+    {
+        SetupLock lk(hip_setup_mutex());
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        (void)hipFree(cnt);
+    }
     if (rc) {
         apply_shape(s, shape0);
         return rc;

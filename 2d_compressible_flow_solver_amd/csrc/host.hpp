@@ -12,11 +12,7 @@
 
 #include "../../include/cfs.h"
 #include "kernels.hpp"
-
-#define HIP_OK(x)                                  \
-    do {                                           \
-        if ((x) != hipSuccess) return CFS_E_HIP;   \
-    } while (0)
+#include "setup_lock.h"
 
 namespace cfs {
 

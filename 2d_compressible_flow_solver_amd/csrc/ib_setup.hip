@@ -42,6 +42,7 @@
 #include <vector>
 
 #include "../../include/cfs.h"
+#include "setup_lock.h"
 
 namespace {
 
@@ -436,10 +437,7 @@ struct DevArena {
 
 inline size_t up256(size_t n) { return (n + 255) & ~(size_t)255; }
 
-#define IB_HIP(x)                                        \
-    do {                                                 \
-        if ((x) != hipSuccess) return CFS_E_HIP;         \
-    } while (0)
+#define IB_HIP(x) HIP_OK(x)
 
 }  // namespace
 
@@ -527,6 +525,9 @@ extern "C" int32_t cfs_circle_setup_device(const cfs_config* cfg, double cx, dou
     const size_t oDon = o;    o += up256(sizeof(int32_t) * 4 * nk);
     const size_t oTr = o;     o += up256(sizeof(int32_t) * 4 * nk);
     const size_t oFlag = o;   o += up256(sizeof(int32_t) * nk);
+    // the set-up's allocation, null-stream work and free, made one at a time
+    // with the library's other set-up calls and stream captures
+    cfs::SetupLock lk(cfs::hip_setup_mutex());
     DevArena mem;
     IB_HIP(hipMalloc(&mem.base, o));
     uint8_t* dT = reinterpret_cast<uint8_t*>(mem.base + oT);

@@ -1326,10 +1326,16 @@ __device__ __forceinline__ void update_cell(const StageArgs& a, long long c, int
 // batched kernel, whose registers do not take both forms of every variable).
 // EARLY: the first staging pass is read before the time step is derived (the
 // batched kernel, which also holds its problem's arguments, spills with it).
+struct NoHook {
+    __device__ void operator()() const {}
+};
+
+// `staged` runs once the tile is staged (after the barrier that ends the
+// staging, before the faces): the batched kernel takes its next tile there.
 template <typename R, bool VISC, bool LIMIT, typename Sh, bool SPLIT = true, bool HALF = false, bool PER_VAR = true,
-          bool EARLY = true>
+          bool EARLY = true, typename Hook = NoHook>
 __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R* F, double* wmax,
-                                           ViscLDS<R, Sh>* V, LimitLDS<R, Sh>* Lm) {
+                                           ViscLDS<R, Sh>* V, LimitLDS<R, Sh>* Lm, const Hook& staged = Hook()) {
     constexpr int TI = Sh::TI, TJ = Sh::TJ, SR = Sh::SR, SC = Sh::SC, NXF = Sh::NXF, BLOCK = Sh::NT;
     constexpr int NS = SR * SC, NF = Sh::template flux_plane<HALF>();   // LDS plane lengths (state, fluxes)
     // y-faces' first slot in the flux planes
@@ -1474,6 +1480,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
         __syncthreads();
     }
     if (sizeof(R) == 8 && tile_unsafe && tid == 0) atomicAdd(a.exact_count, 1);
+    staged();
 
     if constexpr (LIMIT) {
         if (a.positivity == 2) {
@@ -1653,32 +1660,76 @@ __global__ __launch_bounds__(Sh::NT, (Occupancy<R, VISC, LIMIT, Sh>::waves_per_s
 // one per XCD (workgroup b is dispatched to XCD b % 8), so tiles that share
 // halo cells are staged through the same L2.  Every problem's arithmetic is
 // the single-solve kernel's, so a problem's result does not depend on what
-// else is in the batch.  The grid size is a multiple of 8.  (Taking tiles
-// from an atomic counter per range instead of the fixed stride was slower:
-// a range's workgroups reach the counter together.)
+// else is in the batch, nor on which workgroup takes which tile.
+//
+// Tiles are taken in list order, so that the tiles in flight at any time are
+// neighbours in the list and a tile's halo lines are still in the L2 when the
+// next row of tiles reads them (with a fixed stride the workgroups drift
+// rounds apart: 1.19x the algorithmic reads on 64 problems of 512^2, 1.27x on
+// 16 of 1024^2, against 1.01x in list order).  Each range is cut into
+// BATCH_SUBRANGES contiguous parts with a counter each, every counter on a
+// cache line of its own (device-scope atomics on one line are served one at
+// a time: two counters on one line were 7-9% slower than on two); more parts
+// put more windows of tiles in flight in one L2 (eight: 1.28x the reads at
+// 1024^2).  A part's workgroups take its first tiles by their index and the
+// rest from its counter, a tile ahead (the atomic is issued once the current
+// tile is staged, so its latency overlaps the faces).  Counters are kept per
+// (step parity, sub-stage): a launch zeroes those of every other sub-stage,
+// and consecutive launches are always of different sub-stages (a step runs
+// sub-stages 0 .. rk - 1, rk >= 2), so each launch finds its own at zero.
+constexpr int BATCH_SUBRANGES = 2;
+constexpr int BATCH_SCHED_PAD = 32;   // ints per counter: one 128-byte line each
+constexpr int BATCH_SCHED_INTS = 6 * 8 * BATCH_SUBRANGES * BATCH_SCHED_PAD;   // [parity * 3 + sub][range][part]
+
 template <typename R, bool LIMIT, typename Sh>
 __global__ __launch_bounds__(Sh::NT, (Occupancy<R, false, LIMIT, Sh>::batch_waves_per_simd)) void batch_stage_kernel(
     const StageArgs* __restrict__ descs, const int2* __restrict__ tiles, const int* __restrict__ ntiles, int parity,
-    int sub) {
+    int sub, int* __restrict__ sched) {
+    constexpr int NP = BATCH_SUBRANGES, PER_SET = 8 * NP;
     __shared__ R S[Sh::SR * Sh::SC * 4];
     __shared__ R F[(Sh::NXF + Sh::NYF) * 4];
     __shared__ double wmax[Sh::NT / 64];
+    __shared__ int next_tile;
     using LimitT = typename std::conditional<LIMIT, LimitLDS<R, Sh>, char>::type;
     __shared__ LimitT Lm;
+    const int set = parity * 3 + sub;
+    if (blockIdx.x == 0) {   // the other sub-stages' counters, for their launches
+        for (int i = (int)threadIdx.x; i < 5 * PER_SET; i += Sh::NT) {
+            const int o = i / PER_SET, os = o < set ? o : o + 1;
+            atomicExch(&sched[(os * PER_SET + i % PER_SET) * BATCH_SCHED_PAD], 0);
+        }
+    }
     const long long n = *ntiles;
-    const int per = (int)(gridDim.x >> 3);
+    const int per = (int)(gridDim.x >> 3);                 // workgroups of a range
     const int x = (int)(blockIdx.x & 7), l = (int)(blockIdx.x >> 3);
-    const int lo = (int)((n * x) >> 3), hi = (int)((n * (x + 1)) >> 3);
-    for (int t = lo + l; t < hi; t += per) {
+    const int r = l % NP, q = l / NP;                      // part, and rank among the part's workgroups
+    const int wr = (per - r + NP - 1) / NP;                // workgroups of part r
+    const long long lo = (n * x) >> 3, len = ((n * (x + 1)) >> 3) - lo;
+    const int plo = (int)(lo + len * r / NP), phi = (int)(lo + len * (r + 1) / NP);
+    int* ctr = &sched[(set * PER_SET + x * NP + r) * BATCH_SCHED_PAD];
+    const bool counted = plo + wr < phi;                   // the part has tiles beyond the first round
+    const bool lead = threadIdx.x == 0;
+    int t = plo + q;
+    while (t < phi) {
+        int ahead = phi;
+        bool taken = false;
+        auto take = [&]() {
+            if (lead && counted) ahead = plo + wr + atomicAdd(ctr, 1);
+            taken = true;
+        };
         const int2 pt = tiles[t];
         constexpr bool split = Occupancy<R, false, LIMIT, Sh>::batch_split;
         if constexpr (LIMIT)
             stage_tile<R, false, true, Sh, split, false, false, false>(descs[(pt.x * 2 + parity) * 3 + sub], pt.y,
-                                                                       S, F, wmax, nullptr, &Lm);
+                                                                       S, F, wmax, nullptr, &Lm, take);
         else
             stage_tile<R, false, false, Sh, split, false, false, false>(descs[(pt.x * 2 + parity) * 3 + sub], pt.y,
-                                                                        S, F, wmax, nullptr, nullptr);
-        __syncthreads();   // the next tile reuses S, F and wmax
+                                                                        S, F, wmax, nullptr, nullptr, take);
+        if (!taken) take();   // a halted problem's stage returns before its tile is staged
+        __syncthreads();      // the next tile reuses S, F and wmax; next_tile is read
+        if (lead) next_tile = ahead;
+        __syncthreads();
+        t = next_tile;
     }
 }
 

@@ -300,7 +300,7 @@ typedef struct cfs_batch_config {
     int32_t rk;              /* 2 or 3, shared by every problem of the batch */
     int32_t precision;       /* CFS_PREC_FP64 | CFS_PREC_FP32_BULK, shared */
     int32_t max_problems;    /* slot table size */
-    int32_t workgroups;      /* persistent grid; 0: 4 x CUs x resident workgroups per CU */
+    int32_t workgroups;      /* persistent grid; 0: CUs x resident workgroups per CU */
     int64_t arena_bytes;     /* device memory for problem state and tables */
     int32_t limiter;         /* 1: kernel with the positivity limiter (problems may use positivity = 2) */
     int32_t tile_shape;      /* stage-kernel tile shape, 0 .. CFS_NUM_TILE_SHAPES-1
