@@ -1701,11 +1701,12 @@ __global__ __launch_bounds__(Sh::NT, (Occupancy<R, false, LIMIT, Sh>::batch_wave
     }
     const long long n = *ntiles;
     const int per = (int)(gridDim.x >> 3);                 // workgroups of a range
+    const int np = per < NP ? per : NP;                    // parts: at least one workgroup each
     const int x = (int)(blockIdx.x & 7), l = (int)(blockIdx.x >> 3);
-    const int r = l % NP, q = l / NP;                      // part, and rank among the part's workgroups
-    const int wr = (per - r + NP - 1) / NP;                // workgroups of part r
+    const int r = l % np, q = l / np;                      // part, and rank among the part's workgroups
+    const int wr = (per - r + np - 1) / np;                // workgroups of part r
     const long long lo = (n * x) >> 3, len = ((n * (x + 1)) >> 3) - lo;
-    const int plo = (int)(lo + len * r / NP), phi = (int)(lo + len * (r + 1) / NP);
+    const int plo = (int)(lo + len * r / np), phi = (int)(lo + len * (r + 1) / np);
     int* ctr = &sched[(set * PER_SET + x * NP + r) * BATCH_SCHED_PAD];
     const bool counted = plo + wr < phi;                   // the part has tiles beyond the first round
     const bool lead = threadIdx.x == 0;

@@ -77,6 +77,23 @@ def test_batch_equals_alone(precision, rk, positivity):
         assert np.array_equal(Ub.view(np.uint64), ref.view(np.uint64)), "problem %d differs" % k
 
 
+@pytest.mark.parametrize("workgroups", [8, 24, 40])
+def test_batch_small_persistent_grid(workgroups):
+    """A persistent grid of 8, 24 or 40 workgroups (one, three or five per XCD
+    range: fewer than the parts a range is cut into, an odd count, and a
+    count that leaves parts with different numbers of workgroups) still
+    takes every tile of every problem once: the results equal solving alone."""
+    probs = _problems(cfs.PREC_FP64, 2, 2)
+    with cfs.Batch(rk=2, precision=cfs.PREC_FP64, max_problems=16, arena_bytes=256 << 20, limiter=True,
+                   workgroups=workgroups) as b:
+        got = _run_batch(b, probs, first=2)
+    for k, (cfg, U, ib, cfl, t_end) in enumerate(probs):
+        ref, t, done = _alone(cfg, U, ib, cfl, t_end)
+        Ub, tb, steps = got[k]
+        assert steps == done and tb == t
+        assert np.array_equal(Ub.view(np.uint64), ref.view(np.uint64)), "problem %d differs" % k
+
+
 def test_batch_arena_full_then_reuse():
     cfg = cfs.make_config(64, 64, 1.0, 1.0, rk=3)
     U = cfs.riemann_quadrants(cfg)
