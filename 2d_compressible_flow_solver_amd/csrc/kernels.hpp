@@ -1321,19 +1321,21 @@ __device__ __forceinline__ void update_cell(const StageArgs& a, long long c, int
     }
 }
 
-// PER_VAR: a tile outside the fast WENO5 range takes the IEEE form for the
-// variables that hold the out-of-range values only (else for all four: the
-// batched kernel, whose registers do not take both forms of every variable).
-// EARLY: the first staging pass is read before the time step is derived (the
-// batched kernel, which also holds its problem's arguments, spills with it).
 struct NoHook {
     __device__ void operator()() const {}
 };
 
+// PER_VAR: a tile outside the fast WENO5 range takes the IEEE form for the
+// variables that hold the out-of-range values only (else for all four: the
+// batched kernel, whose registers do not take both forms of every variable).
+// PEEL: the first staging pass is peeled off the staging loop, which keeps
+// the FP32-bulk single-solve kernels free of register spills (0.0881 ->
+// 0.0851 ms on the Mach-3 cylinder); the batched kernel, which also holds its
+// problem's arguments, spills with it and keeps the plain loop.
 // `staged` runs once the tile is staged (after the barrier that ends the
 // staging, before the faces): the batched kernel takes its next tile there.
 template <typename R, bool VISC, bool LIMIT, typename Sh, bool SPLIT = true, bool HALF = false, bool PER_VAR = true,
-          bool EARLY = true, typename Hook = NoHook>
+          bool PEEL = true, typename Hook = NoHook>
 __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R* F, double* wmax,
                                            ViscLDS<R, Sh>* V, LimitLDS<R, Sh>* Lm, const Hook& staged = Hook()) {
     constexpr int TI = Sh::TI, TJ = Sh::TJ, SR = Sh::SR, SC = Sh::SC, NXF = Sh::NXF, BLOCK = Sh::NT;
@@ -1397,17 +1399,6 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
                 limit_cell<R, Sh>(s, r - HALO, c - HALO, (R)v[0], (R)v[1], (R)v[2], (R)v[3], (R)a.gamma, F, Lm);
         }
     };
-    // the first pass's loads are issued before the time step is known: the
-    // step's scalars and the tile's cells are read in one memory latency
-    // (nothing is written before the time step decides that the stage runs;
-    // the first pass is peeled off the loop, which also keeps the FP32-bulk
-    // kernels free of register spills)
-    double f0[4], f1[4];
-    long long fc0 = -1, fc1 = -1;
-    const bool fn0 = EARLY && needed(tid), fn1 = EARLY && PER == 2 && tid + BLOCK < SR * SC && needed(tid + BLOCK);
-    if (fn0) fc0 = load(tid, f0);
-    if (fn1) fc1 = load(tid + BLOCK, f1);
-
     // ---- time step ----
     double dt;
     if (a.dt_mode == 1) {
@@ -1449,9 +1440,15 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
         dt = *a.dt;
     }
 
+    // the first staging pass, peeled (after the time step: a halted stage reads nothing)
+    double f0[4], f1[4];
+    long long fc0 = -1, fc1 = -1;
+    const bool fn0 = PEEL && needed(tid), fn1 = PEEL && PER == 2 && tid + BLOCK < SR * SC && needed(tid + BLOCK);
+    if (fn0) fc0 = load(tid, f0);
+    if (fn1) fc1 = load(tid + BLOCK, f1);
     if (fn0) store(tid, f0, fc0);
     if (fn1) store(tid + BLOCK, f1, fc1);
-    for (int s0 = tid + (EARLY ? PER * BLOCK : 0); s0 < SR * SC; s0 += PER * BLOCK) {
+    for (int s0 = tid + (PEEL ? PER * BLOCK : 0); s0 < SR * SC; s0 += PER * BLOCK) {
         const int s1 = s0 + BLOCK;
         const bool n0 = needed(s0), n1 = PER == 2 && s1 < SR * SC && needed(s1);
         double v0[4], v1[4];
