@@ -293,27 +293,81 @@ __device__ __forceinline__ bool fast_div_safe(double v) {
 // with beta'_k = (12/13) beta_k and eps' = (12/13) eps (the common factor
 // (13/12)^2 of the r_k cancels) and D_k = d_k q_k, the ideal weights folded
 // into the candidate stencils' coefficients.
-__device__ __forceinline__ float weno5_f32(float v0, float v1, float v2, float v3, float v4) {
-    constexpr float k = 3.0f / 13.0f, e = 1e-2f * 12.0f / 13.0f;   // (12/13)(1/4) and (12/13) eps
-    const float s0 = __builtin_fmaf(-2.0f, v1, v0) + v2;
-    const float t0 = __builtin_fmaf(3.0f, v2, __builtin_fmaf(-4.0f, v1, v0));
-    const float b0 = __builtin_fmaf(s0, s0, (k * t0) * t0);
-    const float s1 = __builtin_fmaf(-2.0f, v2, v1) + v3;
-    const float t1 = v1 - v3;
-    const float b1 = __builtin_fmaf(s1, s1, (k * t1) * t1);
-    const float s2 = __builtin_fmaf(-2.0f, v3, v2) + v4;
-    const float t2 = __builtin_fmaf(-4.0f, v3, __builtin_fmaf(3.0f, v2, v4));
-    const float b2 = __builtin_fmaf(s2, s2, (k * t2) * t2);
+constexpr float WENO_F32_K = 3.0f / 13.0f;           // (12/13)(1/4)
+constexpr float WENO_F32_EPS = 1e-2f * 12.0f / 13.0f;  // (12/13) eps
+
+// The smoothness indicator beta'_k = s^2 + k t^2 of one candidate stencil.
+__device__ __forceinline__ float weno5_beta_f32(float s, float t) {
+    return __builtin_fmaf(s, s, (WENO_F32_K * t) * t);
+}
+
+// value = (sum_k r_k D_k) / (sum_k d_k r_k), r_k = 1/E_k, E_k = (eps' + beta'_k)^2.
+__device__ __forceinline__ float weno5_combine_f32(float b0, float b1, float b2, float D0, float D1, float D2) {
+    constexpr float e = WENO_F32_EPS;
     const float e0 = e + b0, e1 = e + b1, e2 = e + b2;
-    const float r0 = __builtin_amdgcn_rcpf(e0 * e0);
-    const float r1 = __builtin_amdgcn_rcpf(e1 * e1);
-    const float r2 = __builtin_amdgcn_rcpf(e2 * e2);
-    const float D0 = __builtin_fmaf(0.1f * 11.0f / 6.0f, v2, __builtin_fmaf(-0.1f * 7.0f / 6.0f, v1, v0 * (0.1f / 3.0f)));
-    const float D1 = __builtin_fmaf(0.6f / 3.0f, v3, __builtin_fmaf(0.6f * 5.0f / 6.0f, v2, v1 * (-0.6f / 6.0f)));
-    const float D2 = __builtin_fmaf(-0.3f / 6.0f, v4, __builtin_fmaf(0.3f * 5.0f / 6.0f, v3, v2 * (0.3f / 3.0f)));
+    const float E0 = e0 * e0, E1 = e1 * e1, E2 = e2 * e2;
+    // r_k = 1/E_k multiplied through by E0 E1 E2: r_k -> the product of the
+    // other two E's, one reciprocal instead of four.  E_k >= e^2 > 8e-5, so
+    // with every E_k <= 2^40 the products lie in [7e-9, 2^80] and the
+    // denominator is positive and finite; a numerator beyond the FP32 range
+    // (values above ~1e13) shows as a non-finite result.  Either way out of
+    // range, the reciprocal form.
+    const float m = __builtin_fmaxf(E0, __builtin_fmaxf(E1, E2));
+    const float P12 = E1 * E2, P02 = E0 * E2, P01 = E0 * E1;
+    const float pnum = __builtin_fmaf(P01, D2, __builtin_fmaf(P02, D1, P12 * D0));
+    const float pden = __builtin_fmaf(0.3f, P01, __builtin_fmaf(0.6f, P02, 0.1f * P12));
+    const float val = pnum * __builtin_amdgcn_rcpf(pden);
+    if (m <= 0x1p40f && __builtin_fabsf(val) <= 0x1.fffffep127f) return val;
+    const float r0 = __builtin_amdgcn_rcpf(E0);
+    const float r1 = __builtin_amdgcn_rcpf(E1);
+    const float r2 = __builtin_amdgcn_rcpf(E2);
     const float num = __builtin_fmaf(r2, D2, __builtin_fmaf(r1, D1, r0 * D0));
     const float den = __builtin_fmaf(0.3f, r2, __builtin_fmaf(0.6f, r1, 0.1f * r0));
     return num * __builtin_amdgcn_rcpf(den);
+}
+
+// D_k = d_k q_k of the candidate stencils of (v0 .. v4), v2 the upwind cell.
+__device__ __forceinline__ float weno5_d0_f32(float v0, float v1, float v2) {
+    return __builtin_fmaf(0.1f * 11.0f / 6.0f, v2, __builtin_fmaf(-0.1f * 7.0f / 6.0f, v1, v0 * (0.1f / 3.0f)));
+}
+__device__ __forceinline__ float weno5_d1_f32(float v1, float v2, float v3) {
+    return __builtin_fmaf(0.6f / 3.0f, v3, __builtin_fmaf(0.6f * 5.0f / 6.0f, v2, v1 * (-0.6f / 6.0f)));
+}
+__device__ __forceinline__ float weno5_d2_f32(float v2, float v3, float v4) {
+    return __builtin_fmaf(-0.3f / 6.0f, v4, __builtin_fmaf(0.3f * 5.0f / 6.0f, v3, v2 * (0.3f / 3.0f)));
+}
+
+__device__ __forceinline__ float weno5_f32(float v0, float v1, float v2, float v3, float v4) {
+    const float b0 = weno5_beta_f32(__builtin_fmaf(-2.0f, v1, v0) + v2,
+                                    __builtin_fmaf(3.0f, v2, __builtin_fmaf(-4.0f, v1, v0)));
+    const float b1 = weno5_beta_f32(__builtin_fmaf(-2.0f, v2, v1) + v3, v1 - v3);
+    const float b2 = weno5_beta_f32(__builtin_fmaf(-2.0f, v3, v2) + v4,
+                                    __builtin_fmaf(-4.0f, v3, __builtin_fmaf(3.0f, v2, v4)));
+    return weno5_combine_f32(b0, b1, b2, weno5_d0_f32(v0, v1, v2), weno5_d1_f32(v1, v2, v3),
+                             weno5_d2_f32(v2, v3, v4));
+}
+
+// Both states of a face from its six cells c0 .. c5 (cells f-3 .. f+2): the
+// left state is weno5_f32(c0 .. c4), the right state weno5_f32(c5 .. c1).
+// They share, up to rounding, the second differences of the triples
+// (c1, c2, c3) and (c2, c3, c4) (a - 2b + c is symmetric under reversal) and
+// two candidate values: 0.3 q2(c2, c3, c4) = 0.6 q1(c4, c3, c2) / 2 and
+// 0.3 q2(c3, c2, c1) = 0.6 q1(c1, c2, c3) / 2.
+__device__ __forceinline__ void weno5_pair_f32(const float c[6], float& L, float& Rr) {
+    const float c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3], c4 = c[4], c5 = c[5];
+    const float s0 = __builtin_fmaf(-2.0f, c1, c0) + c2;
+    const float s1 = __builtin_fmaf(-2.0f, c2, c1) + c3;
+    const float s2 = __builtin_fmaf(-2.0f, c3, c2) + c4;
+    const float s3 = __builtin_fmaf(-2.0f, c4, c3) + c5;
+    const float bL0 = weno5_beta_f32(s0, __builtin_fmaf(3.0f, c2, __builtin_fmaf(-4.0f, c1, c0)));
+    const float bL1 = weno5_beta_f32(s1, c1 - c3);
+    const float bL2 = weno5_beta_f32(s2, __builtin_fmaf(-4.0f, c3, __builtin_fmaf(3.0f, c2, c4)));
+    const float bR0 = weno5_beta_f32(s3, __builtin_fmaf(3.0f, c3, __builtin_fmaf(-4.0f, c4, c5)));
+    const float bR1 = weno5_beta_f32(s2, c4 - c2);
+    const float bR2 = weno5_beta_f32(s1, __builtin_fmaf(-4.0f, c2, __builtin_fmaf(3.0f, c3, c1)));
+    const float DL1 = weno5_d1_f32(c1, c2, c3), DR1 = weno5_d1_f32(c4, c3, c2);
+    L = weno5_combine_f32(bL0, bL1, bL2, weno5_d0_f32(c0, c1, c2), DL1, 0.5f * DR1);
+    Rr = weno5_combine_f32(bR0, bR1, bR2, weno5_d0_f32(c5, c4, c3), DR1, 0.5f * DL1);
 }
 
 template <typename R, bool FAST>
@@ -343,6 +397,19 @@ __device__ __forceinline__ R weno5(R v0, R v1, R v2, R v3, R v4) {
     const R q1 = (-D::d6(v1) + D::d6(R(5) * v2)) + D::d3(v3);
     const R q2 = (D::d3(v2) + D::d6(R(5) * v3)) - D::d6(v4);
     return (w0 * q0 + w1 * q1) + w2 * q2;
+}
+
+// Left and right states of one variable from the face's six cells v[0..5]
+// (cells f-3 .. f+2): weno5 of v[0..4] and of v[5..1].  FP32-bulk computes
+// the two together (weno5_pair_f32).
+template <typename R, bool FAST>
+__device__ __forceinline__ void weno5_lr(const R v[6], R& L, R& Rr) {
+    if constexpr (sizeof(R) == 4) {
+        weno5_pair_f32(v, L, Rr);
+    } else {
+        L = weno5<R, FAST>(v[0], v[1], v[2], v[3], v[4]);
+        Rr = weno5<R, FAST>(v[5], v[4], v[3], v[2], v[1]);
+    }
 }
 
 // The divisions of a face state (r, n, t, E) by its density, as the
@@ -403,11 +470,9 @@ __device__ __forceinline__ void face_flux(const R* __restrict__ S, int ns, const
 #pragma unroll
         for (int k = 0; k < 6; ++k) v[k] = S[comp[q] * ns + cell[k]];
         if (!FAST && ((slow >> comp[q]) & 1u)) {   // this variable's IEEE form (uniform per tile)
-            L[q] = weno5<R, false>(v[0], v[1], v[2], v[3], v[4]);
-            Rr[q] = weno5<R, false>(v[5], v[4], v[3], v[2], v[1]);
+            weno5_lr<R, false>(v, L[q], Rr[q]);
         } else {
-            L[q] = weno5<R, true>(v[0], v[1], v[2], v[3], v[4]);
-            Rr[q] = weno5<R, true>(v[5], v[4], v[3], v[2], v[1]);
+            weno5_lr<R, true>(v, L[q], Rr[q]);
         }
         cl[q] = v[2];
         cr[q] = v[3];
@@ -1172,11 +1237,9 @@ __device__ __forceinline__ void tile_face_pair(const StageArgs& a, int s, int i0
 #pragma unroll
             for (int k = 0; k < 6; ++k) v[k] = S[comp[f][q] * NS + cell[f][k]];
             if (!FAST && ((slow >> comp[f][q]) & 1u)) {
-                L[f][q] = weno5<R, false>(v[0], v[1], v[2], v[3], v[4]);
-                Rr[f][q] = weno5<R, false>(v[5], v[4], v[3], v[2], v[1]);
+                weno5_lr<R, false>(v, L[f][q], Rr[f][q]);
             } else {
-                L[f][q] = weno5<R, true>(v[0], v[1], v[2], v[3], v[4]);
-                Rr[f][q] = weno5<R, true>(v[5], v[4], v[3], v[2], v[1]);
+                weno5_lr<R, true>(v, L[f][q], Rr[f][q]);
             }
             cl[f][q] = v[2];
             cr[f][q] = v[3];
