@@ -150,12 +150,15 @@ int launch_batch_step(cfs_batch* b, int parity) {
 // tile order (neighbouring tiles stay adjacent in the list).
 int rebuild_tiles(cfs_batch* b) {
     long long n = 0;
+    for (const Slot& s : b->slots)
+        if (s.used) n += s.ntiles;
+    if (n > b->tiles_cap) return CFS_E_FULL;   // checked before the staging buffer is written
+    n = 0;
     for (int p = 0; p < (int)b->slots.size(); ++p) {
         const Slot& s = b->slots[p];
         if (!s.used) continue;
         for (int t = 0; t < s.ntiles; ++t) b->tiles_host[n++] = make_int2(p, t);
     }
-    if (n > b->tiles_cap) return CFS_E_FULL;
     if (n) HIP_OK(hipMemcpyAsync(b->tiles, b->tiles_host, sizeof(int2) * n, hipMemcpyHostToDevice, b->stream));
     const int nn = (int)n;
     HIP_OK(hipMemcpyAsync(b->ntiles, &nn, sizeof(int), hipMemcpyHostToDevice, b->stream));

@@ -66,6 +66,7 @@ struct cfs_solver {
     hipGraphExec_t graph_exec = nullptr;
     GraphKey graph_key;
     int shape = 0;                              // tile shape of the stage kernels
+    int cus = 0;                                // compute units of the handle's device (0: unknown)
     uint8_t* tile_ib_shape[NUM_SHAPES] = {};    // per-tile IB flags of every shape
 };
 
@@ -112,6 +113,17 @@ void dispatch_stage(const cfs_config& c, int shape, dim3 grid, hipStream_t strea
 
 dim3 stage_grid(const cfs_solver* s) { return dim3((unsigned)(8 * ((s->dev.tiles_i * s->dev.tiles_j + 7) / 8))); }
 
+// Grids of at most 16 tiles per CU read each tile's first staging pass
+// before the time step is known: there every workgroup starts in the first
+// round, so a stage costs one tile's latency chain and the load overlaps the
+// time-step scalars' (Lax-Liu 200^2 in CFL steps: 13.6 -> 12.9 us per stage,
+// 400^2: 22.5 -> 22.3 us).  On larger grids the first round's latency is
+// hidden by the later rounds, while a launch replayed past the end time
+// would read every tile (a 4096^2 grid: 179 instead of 49 us per such launch).
+bool stage_early(const cfs_solver* s) {
+    return s->cus > 0 && (long long)s->dev.tiles_i * s->dev.tiles_j <= 16ll * s->cus;
+}
+
 // The tile shape a new handle starts with: the dual-face shape 3 for
 // inviscid grids of at most 5000 tiles of 15 x 16 (up to about 1100^2
 // cells) without the positivity limiter, where its two chains per thread
@@ -137,6 +149,7 @@ int launch_stage(cfs_solver* s, int counter, bool timed, const StageTime& tm) {
     const cfs_config& c = s->cfg;
     StageArgs a;
     const int next = fill_stage_args(a, s->dev, rk_substage(c.rk, counter), s->step_parity, s->parity, tm);
+    a.stage_early = stage_early(s) ? 1 : 0;
     const dim3 grid = stage_grid(s);
     if (timed) {
         SetupLock lk(hip_setup_mutex());
@@ -251,6 +264,12 @@ int32_t cfs_create(const cfs_config* cfg, void* hip_stream, cfs_solver** out) {
     d.cfg = &s->cfg;
     d.plane = (long long)cfg->nx * cfg->ny;
     apply_shape(s, default_shape(*cfg));
+    {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
+            s->cus = cus;
+    }
     if (hip_stream) {
         s->stream = (hipStream_t)hip_stream;
     } else {
@@ -748,6 +767,7 @@ int32_t cfs_autotune(cfs_solver* s, int32_t reps, int32_t* best, double* ms) {
             apply_shape(s, k);
             StageArgs a;
             fill_stage_args(a, s->dev, 0, s->step_parity, 0, StageTime{});
+            a.stage_early = stage_early(s) ? 1 : 0;
             a.exact_count = cnt;
             a.exact_other = cnt + 1;
             const dim3 grid = stage_grid(s);

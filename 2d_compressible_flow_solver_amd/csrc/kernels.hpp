@@ -93,6 +93,9 @@ struct StageArgs {
     int stream_nt;        // the RK snapshots are read and the new state written with non-temporal
                           // accesses (state planes larger than the last-level cache: their lines
                           // are not reused before eviction, and would push out staged halo lines)
+    int stage_early;      // the first staging pass is read before the time step is known (grids of
+                          // a few tiles per CU, where a stage costs one tile's latency chain; a
+                          // launch replayed past the end time then still reads that pass)
     double dx, dy, inv_dx, inv_dy, gamma;
     double c0, c1, c2;
     int bc[4];
@@ -1462,6 +1465,15 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
                 limit_cell<R, Sh>(s, r - HALO, c - HALO, (R)v[0], (R)v[1], (R)v[2], (R)v[3], (R)a.gamma, F, Lm);
         }
     };
+    // the first staging pass (each thread's first cells), peeled off the loop
+    double f0[4], f1[4];
+    long long fc0 = -1, fc1 = -1;
+    const bool fn0 = PEEL && needed(tid), fn1 = PEEL && PER == 2 && tid + BLOCK < SR * SC && needed(tid + BLOCK);
+    const bool early = PEEL && a.stage_early;
+    if (early) {
+        if (fn0) fc0 = load(tid, f0);
+        if (fn1) fc1 = load(tid + BLOCK, f1);
+    }
     // ---- time step ----
     double dt;
     if (a.dt_mode == 1) {
@@ -1503,12 +1515,12 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
         dt = *a.dt;
     }
 
-    // the first staging pass, peeled (after the time step: a halted stage reads nothing)
-    double f0[4], f1[4];
-    long long fc0 = -1, fc1 = -1;
-    const bool fn0 = PEEL && needed(tid), fn1 = PEEL && PER == 2 && tid + BLOCK < SR * SC && needed(tid + BLOCK);
-    if (fn0) fc0 = load(tid, f0);
-    if (fn1) fc1 = load(tid + BLOCK, f1);
+    // otherwise the first pass is read once the time step is known (a halted
+    // stage reads nothing)
+    if (!early) {
+        if (fn0) fc0 = load(tid, f0);
+        if (fn1) fc1 = load(tid + BLOCK, f1);
+    }
     if (fn0) store(tid, f0, fc0);
     if (fn1) store(tid + BLOCK, f1, fc1);
     for (int s0 = tid + (PEEL ? PER * BLOCK : 0); s0 < SR * SC; s0 += PER * BLOCK) {
