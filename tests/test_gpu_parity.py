@@ -271,6 +271,34 @@ def test_fp32_bulk_l1_density():
     assert l1 <= 1e-4
 
 
+def test_fp32_bulk_l1_density_large_values():
+    """The Lax-Liu flow with velocities x 100 and pressures x 1e4 (the same
+    flow on a time scale 100 times shorter, but for WENO5's unscaled eps):
+    energy jumps of ~3e4 make the FP32 WENO5 smoothness terms (eps + beta)^2
+    exceed 2^40 at the discontinuities, where the one-reciprocal product form
+    of the weights hands over to the reciprocal form, while smooth regions
+    keep the product form."""
+    a = 1e2
+    states = {k: (r, a * u, a * v, a * a * p) for k, (r, u, v, p) in cfs.LAX_LIU_3.items()}
+    cfg64 = cfs.config_riemann(64, rk=3)
+    cfg32 = cfs.config_riemann(64, rk=3, precision=cfs.PREC_FP32_BULK)
+    U = cfs.riemann_quadrants(cfg64, states=states)
+    assert np.abs(U[3]).max() > 3e4
+    s64, s32 = cfs.Solver(cfg64), cfs.Solver(cfg32)
+    s64.set_state(U)
+    s32.set_state(U)
+    dt = s64.compute_dt(0.5)
+    for n in range(100):
+        s64.step(dt)
+        s32.step(dt)
+    r64 = s64.get_state()[0]
+    r32 = s32.get_state()[0]
+    assert np.all(np.isfinite(r32))
+    l1 = np.abs(r32 - r64).mean()
+    print("FP32-bulk L1(rho) after 100 steps, velocities x 100, pressures x 1e4:", l1)
+    assert l1 <= 1e-4
+
+
 @pytest.mark.parametrize("rk", [3, 2])
 def test_riemann_400_full_size_100_steps(rk):
     # the Lax-Liu configuration at its full 400^2 size: 100 steps at the CFL
