@@ -4,7 +4,7 @@
 //   cfs_run [--case cylinder|riemann|lax-liu|kh] [--nx N] [--ny N] [--lx L] [--ly L]
 //           [--rk 2|3] [--fp32] [--stages S] [--dt DT | --cfl C --t-end T]
 //           [--every K] [--out DIR] [--positivity] [--velocity-bounds]
-//           [--mu MU] [--radius R] [--cx X] [--cy Y] [--ib-reference]
+//           [--mu MU] [--radius R] [--cx X] [--cy Y] [--ib-reference] [--log FILE]
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -19,7 +19,7 @@ static void usage() {
                  "usage: cfs_run [--case cylinder|riemann|lax-liu|kh] [--nx N] [--ny N] [--lx L] [--ly L]\n"
                  "               [--rk 2|3] [--fp32] [--stages S] [--dt DT | --cfl C --t-end T] [--every K]\n"
                  "               [--out DIR] [--positivity] [--velocity-bounds] [--mu MU]\n"
-                 "               [--radius R] [--cx X] [--cy Y] [--ib-reference]\n");
+                 "               [--radius R] [--cx X] [--cy Y] [--ib-reference] [--log FILE]\n");
 }
 
 int main(int argc, char** argv) {
@@ -45,6 +45,7 @@ int main(int argc, char** argv) {
         else if (k == "--t-end") o.t_end = std::atof(val());
         else if (k == "--every") o.output_every = std::atoi(val());
         else if (k == "--out") o.output_dir = val();
+        else if (k == "--log") o.log_path = val();
         else if (k == "--positivity") o.interpolation.positivity = true;
         else if (k == "--velocity-bounds") o.riemann.velocity_bounds = true;
         else if (k == "--mu") { o.viscous.on = true; o.viscous.mu = std::atof(val()); }

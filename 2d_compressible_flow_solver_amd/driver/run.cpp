@@ -3,6 +3,8 @@
 
 #include <sys/stat.h>
 
+#include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <string>
 
@@ -10,7 +12,43 @@
 
 namespace cfsdrv {
 
-Run::~Run() { cfs_destroy(h_); }
+Run::~Run() {
+    if (log_) std::fclose(log_);
+    cfs_destroy(h_);
+}
+
+namespace {
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+}  // namespace
+
+int Run::log_record(int step) {
+    if (!log_) return CFS_OK;
+    double sums[4];
+    const int rc = cfs_checksums(h_, sums);   // returns once the stages launched so far have ended
+    if (rc) return rc;
+    const double w = now_s();
+    const double dw = w - wall_last_;
+    const int ds = stages_ - stages_last_;
+    const double cells = (double)cfg_.nx * cfg_.ny;
+    // algorithmic bytes per cell-stage: the RK snapshots read and the state
+    // written (RK3 mean 96 B, RK2 80 B), +1 B of cell types with a body
+    const double bytes = (cfg_.rk == 3 ? 96.0 : 80.0) + (cell_type_.empty() ? 0.0 : 1.0);
+    const double mcups = dw > 0.0 ? cells * ds / dw / 1e6 : 0.0;
+    bool finite = true;
+    for (double v : sums) finite = finite && std::isfinite(v);
+    std::fprintf(log_,
+                 "{\"step\": %d, \"stage\": %d, \"t\": %.17g, \"wall_s\": %.6f, \"interval_s\": %.6f, "
+                 "\"interval_stages\": %d, \"mcell_updates_per_s\": %.3f, \"algorithmic_gb_per_s\": %.3f, "
+                 "\"checksums\": [%.17g, %.17g, %.17g, %.17g], \"finite\": %s}\n",
+                 step, stages_, t_, w - wall0_, dw, ds, mcups, mcups * bytes / 1e3, sums[0], sums[1], sums[2],
+                 sums[3], finite ? "true" : "false");
+    std::fflush(log_);
+    wall_last_ = w;
+    stages_last_ = stages_;
+    return CFS_OK;
+}
 
 int Run::initialize() {
     // mesh (Run.cpp:14-20) and plugin choices (Run.cpp:41-52)
@@ -78,16 +116,21 @@ int Run::solver() {
     // uploads and fills the immersed ghosts (Run.cpp:106-112)
     int rc = cfs_set_state(h_, U.data());
     if (rc) return rc;
+    if (!opt_.log_path.empty() && !(log_ = std::fopen(opt_.log_path.c_str(), "w"))) return CFS_E_ARG;
     if (opt_.dt > 0.0) {
         // the reference's loop: one stage per iteration, fixed dt (Run.cpp:153-204)
+        wall0_ = wall_last_ = now_s();
         for (int i = 1; i < opt_.stages; ++i) {
             rc = cfs_stage(h_, i, opt_.dt);
             if (rc) return rc;
             ++stages_;
-            if (opt_.output_every > 0 && i % opt_.output_every == 0 && (rc = snapshot(i))) return rc;
+            if (opt_.output_every > 0 && i % opt_.output_every == 0) {
+                t_ = opt_.dt * stages_ / cfg_.rk;
+                if ((rc = log_record(stages_ / cfg_.rk)) || (rc = snapshot(i))) return rc;
+            }
         }
         t_ = opt_.dt * stages_ / cfg_.rk;
-        return CFS_OK;
+        return stages_ != stages_last_ ? log_record(stages_ / cfg_.rk) : CFS_OK;
     }
     // CFL steps on the device to t_end, snapshots every output_every steps;
     // the stage kernel's tile shape is tuned on the initial state first (the
@@ -95,12 +138,14 @@ int Run::solver() {
     if ((rc = cfs_autotune(h_, 10, nullptr, nullptr))) return rc;
     const int chunk = opt_.output_every > 0 ? opt_.output_every : 1000000;
     int k = 0;
+    wall0_ = wall_last_ = now_s();
     while (opt_.t_end > 0.0 && t_ < opt_.t_end) {
         int32_t done = 0;
         rc = cfs_advance(h_, chunk, 0.0, opt_.cfl, opt_.t_end, &t_, &done);
         stages_ += done * cfg_.rk;
         if (rc) return rc;
         ++k;
+        if ((rc = log_record(stages_ / cfg_.rk))) return rc;
         if (opt_.output_every > 0 && (rc = snapshot(k))) return rc;
         if (done < chunk) break;
     }

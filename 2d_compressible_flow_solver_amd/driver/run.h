@@ -3,6 +3,7 @@
 // behind include/cfs.h.
 #pragma once
 
+#include <cstdio>
 #include <memory>
 #include <string>
 #include <vector>
@@ -31,6 +32,8 @@ struct RunOptions {
     double t_end = 0.0;          // CFL mode: end time
     int output_every = OutputEvery;   // stages (fixed dt) or steps (CFL); 0: none
     std::string output_dir;      // empty: no snapshots
+    std::string log_path;        // JSON-lines run log (the reference's per-stage timing print,
+                                 // Run.cpp:164-232); empty: none
 };
 
 class Run {
@@ -48,12 +51,18 @@ public:
 
 private:
     int snapshot(int index);
+    // one run-log record: progress since the previous record, checksums of
+    // the current state (the checksum call also waits for the stages to end)
+    int log_record(int step);
     RunOptions opt_;
     cfs_config cfg_{};
     cfs_solver* h_ = nullptr;
     std::vector<uint8_t> cell_type_;
     double t_ = 0.0;
     int stages_ = 0;
+    std::FILE* log_ = nullptr;
+    double wall0_ = 0.0, wall_last_ = 0.0;   // seconds: start of the stage loop, previous record
+    int stages_last_ = 0;
 };
 
 }  // namespace cfsdrv

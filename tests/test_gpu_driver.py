@@ -42,11 +42,15 @@ def test_driver_lax_liu_snapshots(tmp_path):
     o.set_state(U)
     dt = 0.5 / o.cfl_smax()
     out = str(tmp_path / "run")
+    log = str(tmp_path / "run.jsonl")
     r = subprocess.run([RUN, "--case", "lax-liu", "--nx", "40", "--ny", "40", "--rk", "2", "--stages", "101",
-                        "--dt", repr(dt), "--every", "50", "--out", out], capture_output=True, text=True,
-                       timeout=120)
+                        "--dt", repr(dt), "--every", "50", "--out", out, "--log", log], capture_output=True,
+                       text=True, timeout=120)
     print(r.stdout, r.stderr)
     assert r.returncode == 0
+    # run log: one record per output interval, the checksums of the state then
+    records = [json.loads(line) for line in open(log)]
+    assert [rec["stage"] for rec in records] == [50, 100]
     s = cfs.Solver(cfg)
     s.set_state(U)
     for i in range(1, 101):
@@ -54,9 +58,13 @@ def test_driver_lax_liu_snapshots(tmp_path):
         o.stage(i, dt)
         if i % 50:
             continue
+        rec = records[i // 50 - 1]
+        assert rec["step"] == i // 2 and rec["interval_stages"] == 50 and rec["finite"] is True
+        assert rec["t"] == dt * i / 2 and rec["mcell_updates_per_s"] > 0 and rec["interval_s"] > 0
+        assert np.array_equal(np.array(rec["checksums"]), s.checksums()), "run-log checksums at stage %d" % i
         snap = _load(os.path.join(out, "snapshot_%06d" % i))
         meta = json.load(open(os.path.join(out, "snapshot_%06d" % i, "meta.json")))
-        assert meta["stage"] == i and meta["nx"] == 40
+        assert meta["stage"] == i and meta["nx"] == 40 and meta["t"] == rec["t"]
         V, R = s.get_state(), s.last_rhs()
         for q, (name, flux) in enumerate((("Rho", "FluxRho"), ("Mu", "FluxMu"), ("Mv", "FluxMv"),
                                           ("Energy", "FluxEnergy"))):
@@ -93,3 +101,27 @@ def test_driver_cylinder_default_geometry(tmp_path):
     assert np.array_equal(snap["FluxMu"], R[1])
     solid = ib["cell_type"].reshape(n, n) == cfs.CELL_SOLID
     assert not snap["Pressure"][solid].any() and not R[:, solid].any()
+
+
+def test_driver_cfl_run_log(tmp_path):
+    # CFL steps to an end time in chunks of --every steps: one run-log record
+    # per chunk, whose time, steps and checksums are those of the same chunked
+    # advance through the Python path
+    log = str(tmp_path / "cfl.jsonl")
+    r = subprocess.run([RUN, "--case", "lax-liu", "--nx", "40", "--ny", "40", "--rk", "2", "--cfl", "0.5",
+                        "--t-end", "0.05", "--every", "10", "--log", log], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    records = [json.loads(line) for line in open(log)]
+    cfg = cfs.config_riemann(40, rk=2)
+    s = cfs.Solver(cfg)
+    s.set_state(cfs.riemann_quadrants(cfg))
+    t, steps = 0.0, 0
+    for rec in records:
+        t, done = s.advance(10, dt=0.0, cfl=0.5, t_end=0.05, t=t)
+        steps += done
+        assert rec["step"] == steps and rec["stage"] == 2 * steps and rec["t"] == t and rec["finite"] is True
+        assert np.array_equal(np.array(rec["checksums"]), s.checksums())
+    assert len(records) >= 2 and abs(records[-1]["t"] - 0.05) <= 1e-15
+    assert json.loads(r.stdout)["t"] == records[-1]["t"]
