@@ -469,9 +469,14 @@ class Solver:
         _check(self._lib.cfs_compute_dt(self._h, cfl, ctypes.byref(dt)), "cfs_compute_dt")
         return dt.value
 
-    def checksums(self):
+    def checksums(self, check_finite=True):
+        """Fixed-order sums of (rho, rho*u, rho*v, E) over fluid cells.  A
+        non-finite sum raises CfsError (CFS_E_NONFINITE) unless check_finite
+        is False, which returns the sums as they are."""
         out = np.zeros(4)
-        _check(self._lib.cfs_checksums(self._h, _ptr(out)), "cfs_checksums")
+        rc = self._lib.cfs_checksums(self._h, _ptr(out))
+        if rc != CFS_E_NONFINITE or check_finite:
+            _check(rc, "cfs_checksums")
         return out
 
     def set_kernel_timing(self, on=True):

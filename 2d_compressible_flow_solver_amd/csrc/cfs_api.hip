@@ -592,8 +592,12 @@ int32_t cfs_checksums(cfs_solver* s, double out[4]) {
     double tmp[4];
     HIP_OK(hipMemcpyAsync(tmp, src, sizeof(double) * 4, hipMemcpyDeviceToHost, s->stream));
     HIP_OK(hipStreamSynchronize(s->stream));
-    for (int q = 0; q < 4; ++q) out[q] = tmp[q];
-    return CFS_OK;
+    int rc = CFS_OK;
+    for (int q = 0; q < 4; ++q) {
+        out[q] = tmp[q];
+        if (!std::isfinite(tmp[q])) rc = CFS_E_NONFINITE;   // a NaN or Inf somewhere in the state
+    }
+    return rc;
 }
 
 }  // extern "C"

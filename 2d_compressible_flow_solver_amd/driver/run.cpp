@@ -26,8 +26,10 @@ double now_s() {
 int Run::log_record(int step) {
     if (!log_) return CFS_OK;
     double sums[4];
-    const int rc = cfs_checksums(h_, sums);   // returns once the stages launched so far have ended
-    if (rc) return rc;
+    // returns once the stages launched so far have ended; a non-finite state
+    // is logged (finite: false) and then ends the run
+    const int rc = cfs_checksums(h_, sums);
+    if (rc && rc != CFS_E_NONFINITE) return rc;
     const double w = now_s();
     const double dw = w - wall_last_;
     const int ds = stages_ - stages_last_;
@@ -47,7 +49,7 @@ int Run::log_record(int step) {
     std::fflush(log_);
     wall_last_ = w;
     stages_last_ = stages_;
-    return CFS_OK;
+    return rc;
 }
 
 int Run::initialize() {

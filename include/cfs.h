@@ -206,7 +206,9 @@ int32_t cfs_advance(cfs_solver* s, int32_t nsteps, double dt, double cfl,
  * cfs_advance).  FP64 max reduction; CFS_E_NONFINITE if s_max is not finite
  * or not positive. */
 int32_t cfs_compute_dt(cfs_solver* s, double cfl, double* dt_out);
-/* Fixed-order FP64 tree sums of (rho, rho*u, rho*v, E) over fluid cells. */
+/* Fixed-order FP64 tree sums of (rho, rho*u, rho*v, E) over fluid cells.
+ * A sum that is not finite (a NaN or Inf in the state) is written to out and
+ * reported as CFS_E_NONFINITE. */
 int32_t cfs_checksums(cfs_solver* s, double out[4]);
 /* cfs_checksums of a state that the caller holds in device memory, with no
  * handle and no device allocation: the tree's partial sums live in a fixed
@@ -217,7 +219,8 @@ int32_t cfs_checksums(cfs_solver* s, double out[4]);
  *   dU     [4][cells] FP64 state on the current device (x-major planes)
  *   d_type [cells] cell types (CFS_CELL_*; only fluid cells are summed), or NULL
  *   d_out  4 doubles of device memory that receive the sums of
- *          (rho, rho*u, rho*v, E), the same bits as cfs_checksums
+ *          (rho, rho*u, rho*v, E), the same bits as cfs_checksums (the call
+ *          does not wait for them, so it cannot report non-finite sums)
  *   hip_stream  stream the kernels are queued on (NULL: the null stream);
  *          the call returns without waiting.
  * cells must lie in [1, CFS_NOALLOC_MAX_CELLS]: larger inputs are CFS_E_LIMIT
