@@ -161,9 +161,21 @@ def test_cfl_dt_and_advance():
     assert done == 20
     assert t == tt
     _assert_parity(s.get_state(), o.get_state(), "cfl advance")
-    # end-time clipping stops exactly at t_end
-    t2, done2 = s.advance(1000, dt=0.0, cfl=0.5, t_end=t + 0.01, t=t)
-    assert abs(t2 - (t + 0.01)) <= 1e-15 and done2 < 1000
+    # end-time clipping stops exactly at t_end; the launches replayed after it
+    # (up to the next read of the halt flag) leave the state as it was: it
+    # equals the oracle's stepped with the same clipped time steps
+    t_end = t + 0.01
+    t2, done2 = s.advance(1000, dt=0.0, cfl=0.5, t_end=t_end, t=t)
+    assert abs(t2 - t_end) <= 1e-15 and done2 < 1000
+    for n in range(done2):
+        dt = 0.5 / o.cfl_smax()
+        rem = t_end - tt
+        if not (dt < rem):
+            dt = rem if rem > 0.0 else 0.0
+        o.step(dt)
+        tt += dt
+    assert tt == t2
+    _assert_parity(s.get_state(), o.get_state(), "cfl advance to an end time")
 
 
 def test_cylinder_cfl_advance_fused_ib_and_bound():
