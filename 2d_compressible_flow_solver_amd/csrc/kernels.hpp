@@ -134,12 +134,27 @@ struct StageArgs {
 // 1 .. TI + 4 of the tile's TJ columns (index (row - 1) * TJ + col - HALO);
 // likewise d/dx for the tile's TI rows and staged columns 1 .. TJ + 4 (index
 // (row - HALO) * (TJ + 4) + col - 1).
+//
+// SHARED (FP64, whose viscous kernel computes the x-faces, then the y-faces
+// over the same flux planes): only x-faces read d/dy and only y-faces d/dx,
+// so d/dx is computed once the x-faces are done, over d/dy: 38 KB of LDS per
+// 15 x 16 tile instead of 52 KB, four workgroups per CU instead of three.
 template <typename R, typename Sh> struct ViscLDS {
     static constexpr int N = Sh::SR * Sh::SC;
     static constexpr int GYW = Sh::TJ, NGY = (Sh::TI + 4) * Sh::TJ;        // d/dy: rows x columns
     static constexpr int GXW = Sh::TJ + 4, NGX = Sh::TI * (Sh::TJ + 4);    // d/dx
+    static constexpr bool SHARED = sizeof(R) == 8;
+    static constexpr int NG = SHARED ? (NGY > NGX ? NGY : NGX) : NGY + NGX;
     R pu[N], pv[N], pT[N];
-    R gyu[NGY], gyv[NGY], gxu[NGX], gxv[NGX];
+    R gu[NG], gv[NG];   // d/dy of u, v at [0, NGY); d/dx at [0, NGX) if SHARED, else at [NGY, NGY + NGX)
+    __device__ R* gyu() { return gu; }
+    __device__ R* gyv() { return gv; }
+    __device__ R* gxu() { return gu + (SHARED ? 0 : NGY); }
+    __device__ R* gxv() { return gv + (SHARED ? 0 : NGY); }
+    __device__ const R* gyu() const { return gu; }
+    __device__ const R* gyv() const { return gv; }
+    __device__ const R* gxu() const { return gu + (SHARED ? 0 : NGY); }
+    __device__ const R* gxv() const { return gv + (SHARED ? 0 : NGY); }
 };
 
 // ---------------------------------------------------------------------------
@@ -1085,8 +1100,8 @@ __device__ __forceinline__ void viscous_flux(const ViscLDS<R, Sh>& V, const int 
                                              R h, bool h_pow2, R inv_h, R mu, R kappa, R out[3]) {
     const R* un = xface ? V.pu : V.pv;
     const R* ut = xface ? V.pv : V.pu;
-    const R* gn = xface ? V.gyu : V.gxv;   // tangential derivative of the normal velocity
-    const R* gt = xface ? V.gyv : V.gxu;   // tangential derivative of the tangential velocity
+    const R* gn = xface ? V.gyu() : V.gxv();   // tangential derivative of the normal velocity
+    const R* gt = xface ? V.gyv() : V.gxu();   // tangential derivative of the tangential velocity
     R acc[3][3];
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
@@ -1290,9 +1305,46 @@ __device__ __forceinline__ R rhs_x(const StageArgs& a, const R* F, int q, int il
 // SPLIT): one flux plane-set of NXF faces, which the y-faces overwrite after
 // each cell has taken the x part of its divergence into tx (the cells of
 // threads tid < TI * TJ).
+// 4th-order central cell gradients of u and v where the faces read them,
+// entries [q0, q1) of d/dy ([0, NGY)) followed by d/dx ([NGY, NGY + NGX))
+// (FourthOrderCS.cpp:120-151 stencil).  The 5-cell stencils stay inside the
+// staged region: d/dy columns HALO +- 2 .. HALO + TJ - 1 + 2, d/dx rows
+// likewise.
+template <typename R, typename Sh>
+__device__ __forceinline__ void visc_gradients(const StageArgs& a, ViscLDS<R, Sh>* V, int q0, int q1) {
+    using VL = ViscLDS<R, Sh>;
+    constexpr int SC = Sh::SC;
+    const R rdx = (R)a.dx, rdy = (R)a.dy;
+    for (int q = q0 + (int)threadIdx.x; q < q1; q += Sh::NT) {
+        R du = R(0), dv = R(0);
+        if (q < VL::NGY) {
+            const int gr = q / VL::GYW, gc = q - gr * VL::GYW;
+            const int s = (gr + 1) * SC + gc + HALO;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                du += VT<R>::thc[k] * V->pu[s - 2 + k];
+                dv += VT<R>::thc[k] * V->pv[s - 2 + k];
+            }
+            V->gyu()[q] = a.dy_pow2 ? du * (R)a.inv_dy : K<R>::div(du, rdy);
+            V->gyv()[q] = a.dy_pow2 ? dv * (R)a.inv_dy : K<R>::div(dv, rdy);
+        } else {
+            const int t = q - VL::NGY;
+            const int gr = t / VL::GXW, gc = t - gr * VL::GXW;
+            const int s = (gr + HALO) * SC + gc + 1;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                du += VT<R>::thc[k] * V->pu[s + (k - 2) * SC];
+                dv += VT<R>::thc[k] * V->pv[s + (k - 2) * SC];
+            }
+            V->gxu()[t] = a.dx_pow2 ? du * (R)a.inv_dx : K<R>::div(du, rdx);
+            V->gxv()[t] = a.dx_pow2 ? dv * (R)a.inv_dx : K<R>::div(dv, rdx);
+        }
+    }
+}
+
 template <typename R, bool VISC, bool LIMIT, bool FAST, typename Sh, bool SPLIT, bool HALF>
 __device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, double dt, unsigned slow, const R* S, R* F,
-                                           const ViscLDS<R, Sh>* V, const LimitLDS<R, Sh>* Lm, R tx[4]) {
+                                           ViscLDS<R, Sh>* V, const LimitLDS<R, Sh>* Lm, R tx[4]) {
     static_assert(!HALF || SPLIT, "the halved flux planes need the x-faces first");
     constexpr int NXF = Sh::NXF, NF = Sh::NXF + Sh::NYF;
     const int tid = threadIdx.x;
@@ -1326,6 +1378,8 @@ __device__ __forceinline__ void tile_faces(const StageArgs& a, int i0, int j0, d
 #pragma unroll
                 for (int q = 0; q < 4; ++q) tx[q] = rhs_x<R, Sh, HALF>(a, F, q, il, jl);
             }
+            if constexpr (VISC && ViscLDS<R, Sh>::SHARED)   // d/dx over d/dy, which the x-faces have read
+                visc_gradients<R, Sh>(a, V, ViscLDS<R, Sh>::NGY, ViscLDS<R, Sh>::NGY + ViscLDS<R, Sh>::NGX);
             __syncthreads();
         }
         if (yreg) {
@@ -1574,36 +1628,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
             V->pT[s] = K<R>::div(gma2 * p, rho);
         }
         __syncthreads();
-        // 4th-order central cell gradients where the faces read them (the
-        // 5-cell stencils stay inside the staged region: d/dy columns
-        // HALO +- 2 .. HALO + TJ - 1 + 2, d/dx rows likewise)
+        // cell gradients where the faces read them: d/dy and d/dx, or with
+        // shared storage d/dy now and d/dx once the x-faces are done
         using VL = ViscLDS<R, Sh>;
-        const R rdx = (R)a.dx, rdy = (R)a.dy;
-        for (int q = tid; q < VL::NGY + VL::NGX; q += BLOCK) {
-            R du = R(0), dv = R(0);
-            if (q < VL::NGY) {
-                const int gr = q / VL::GYW, gc = q - gr * VL::GYW;
-                const int s = (gr + 1) * SC + gc + HALO;
-#pragma unroll
-                for (int k = 0; k < 5; ++k) {
-                    du += VT<R>::thc[k] * V->pu[s - 2 + k];
-                    dv += VT<R>::thc[k] * V->pv[s - 2 + k];
-                }
-                V->gyu[q] = a.dy_pow2 ? du * (R)a.inv_dy : K<R>::div(du, rdy);
-                V->gyv[q] = a.dy_pow2 ? dv * (R)a.inv_dy : K<R>::div(dv, rdy);
-            } else {
-                const int t = q - VL::NGY;
-                const int gr = t / VL::GXW, gc = t - gr * VL::GXW;
-                const int s = (gr + HALO) * SC + gc + 1;
-#pragma unroll
-                for (int k = 0; k < 5; ++k) {
-                    du += VT<R>::thc[k] * V->pu[s + (k - 2) * SC];
-                    dv += VT<R>::thc[k] * V->pv[s + (k - 2) * SC];
-                }
-                V->gxu[t] = a.dx_pow2 ? du * (R)a.inv_dx : K<R>::div(du, rdx);
-                V->gxv[t] = a.dx_pow2 ? dv * (R)a.inv_dx : K<R>::div(dv, rdx);
-            }
-        }
+        static_assert(!VL::SHARED || HALF, "shared gradient storage needs the x-faces first");
+        visc_gradients<R, Sh>(a, V, 0, VL::SHARED ? VL::NGY : VL::NGY + VL::NGX);
         __syncthreads();
     }
 
@@ -1671,14 +1700,16 @@ __device__ __forceinline__ void stage_tile(const StageArgs& a, int tile, R* S, R
 // __launch_bounds__ is the minimum number of waves per execution unit (SIMD)
 // on this target, which sets the VGPR budget (512 / waves).  The wanted
 // figure -- FP64 5, FP32 6, with the limiter FP64 5, viscous FP64
-// 3 (118 VGPRs) and FP32 6 -- is capped by the waves per SIMD that the
+// 4 and FP32 6 -- is capped by the waves per SIMD that the
 // workgroup's LDS (160 KB per CU) allows.
 template <typename R, bool VISC, bool LIMIT, typename Sh> struct Occupancy {
-    // the single-solve FP64 limiter kernel keeps one flux plane-set of x-faces
-    // (HALF): 31.5 KB instead of 39.5 KB of LDS per 15 x 16 tile, five
-    // workgroups per CU instead of four
-    static constexpr bool half_flux = sizeof(R) == 8 && LIMIT && !VISC && !Sh::DUAL;
-    static constexpr int want = VISC ? (sizeof(R) == 8 ? 3 : 6)
+    // the single-solve FP64 limiter and viscous kernels keep one flux
+    // plane-set of x-faces (HALF): with the limiter 31.5 KB instead of 39.5 KB
+    // of LDS per 15 x 16 tile, five workgroups per CU instead of four;
+    // viscous (its gradient planes shared too) 38 KB instead of 52 KB, four
+    // instead of three
+    static constexpr bool half_flux = sizeof(R) == 8 && (LIMIT || VISC) && !Sh::DUAL;
+    static constexpr int want = VISC ? (sizeof(R) == 8 ? 4 : 6)
                               : Sh::DUAL ? ((LIMIT && sizeof(R) == 8) ? 3 : 4)
                               : LIMIT ? (sizeof(R) == 8 ? 5 : 6) : (sizeof(R) == 8 ? 5 : 6);
     static constexpr int extra = VISC ? (int)sizeof(ViscLDS<R, Sh>) : LIMIT ? (int)sizeof(LimitLDS<R, Sh>) : 0;
@@ -1699,7 +1730,7 @@ template <typename R, bool VISC, bool LIMIT, typename Sh> struct Occupancy {
 
 // The stage kernel: one workgroup per tile.  With 15 x 16 tiles: FP64 at
 // most 96 VGPRs so that five workgroups (20 waves) share a CU; FP32 six; the
-// viscous variant's LDS (52 KB in FP64) allows three, the limiter's (30 KB, HALF) five.
+// viscous variant's LDS (38 KB in FP64, HALF) allows four, the limiter's (31.5 KB, HALF) five.
 //
 // Tile order: workgroups are dispatched round-robin over the 8 XCDs
 // (workgroup b to XCD b % 8), each with its own L2.  The grid is 8 x ceil(n/8)
@@ -1717,7 +1748,7 @@ __global__ __launch_bounds__(Sh::NT, (Occupancy<R, VISC, LIMIT, Sh>::waves_per_s
     if (tile >= a.ntiles) return;
     if constexpr (VISC) {
         __shared__ ViscLDS<R, Sh> V;
-        stage_tile<R, true, false, Sh>(a, tile, S, F, wmax, &V, nullptr);
+        stage_tile<R, true, false, Sh, true, HALF>(a, tile, S, F, wmax, &V, nullptr);
     } else if constexpr (LIMIT) {
         __shared__ LimitLDS<R, Sh> Lm;
         stage_tile<R, false, true, Sh, true, HALF>(a, tile, S, F, wmax, nullptr, &Lm);
