@@ -34,8 +34,8 @@ bounded number of steps; the line names the host's cores and CPU model.
 Beside the headline the line carries the rest of the metric:
   fp32_bulk    FP32-bulk throughput, and its L1(rho) against FP64 at the
                workload's own horizon (the time FP64 reaches in 2000 steps,
-               t = 0.108) whatever --steps is, beside an FP64 control run from
-               a density perturbed by 1e-7 and the 1e-4 target;
+               t = 0.108) whatever --steps is, beside FP64 control runs from
+               a density perturbed by 1e-7 and by 1e-12 and the 1e-4 target;
   kh           20 steps of the 4096^2 viscous shear layer in FP64 and
                FP32-bulk: state planes of 537 MB, beyond the Infinity Cache,
                so its stage kernel streams from HBM (roofline object);
@@ -286,18 +286,25 @@ def density_after_steps(precision, nsteps):
 def run_accuracy():
     """FP32-bulk against FP64 on the headline flow at the workload's own
     horizon, whatever the timed window's length: FP64 takes CONTRACT_STEPS CFL
-    steps from the initial state; FP32-bulk, and as a control FP64 from a
-    density perturbed at the FP32 rounding level (1e-7), advance to exactly
-    the time FP64 reached (their own CFL steps, the last one clipped)."""
+    steps from the initial state; FP32-bulk, and as controls FP64 from a
+    density perturbed at the FP32 rounding level (1e-7) and far below it
+    (1e-12), advance to exactly the time FP64 reached (their own CFL steps,
+    the last one clipped).  The controls measure the flow's own sensitivity:
+    what any perturbation of that size grows to by the horizon."""
     rho64, t64, n64 = density_after_steps(cfs.PREC_FP64, CONTRACT_STEPS)
     rho32, _, n32 = density_at_time(cfs.PREC_FP32_BULK, t64)
     rhoc, _, _ = density_at_time(cfs.PREC_FP64, t64, perturb=1e-7)
+    rhoc12, _, _ = density_at_time(cfs.PREC_FP64, t64, perturb=1e-12)
     l1, l1c = float(np.abs(rho32 - rho64).mean()), float(np.abs(rhoc - rho64).mean())
-    print("fp32-bulk vs fp64 at t = %.5f: L1(rho) %.3e (control %.3e)" % (t64, l1, l1c), file=sys.stderr, flush=True)
+    l1c12 = float(np.abs(rhoc12 - rho64).mean())
+    print("fp32-bulk vs fp64 at t = %.5f: L1(rho) %.3e (controls: 1e-7 %.3e, 1e-12 %.3e)" % (t64, l1, l1c, l1c12),
+          file=sys.stderr, flush=True)
     return {"compared_at_time": t64, "steps_fp64": n64, "steps_fp32_bulk": n32,
             "l1_density_vs_fp64": l1, "linf_density_vs_fp64": float(np.abs(rho32 - rho64).max()),
             "control_fp64_density_perturbed_1e-7": {"l1_density_vs_fp64": l1c,
                                                     "linf_density_vs_fp64": float(np.abs(rhoc - rho64).max())},
+            "control_fp64_density_perturbed_1e-12": {"l1_density_vs_fp64": l1c12,
+                                                     "linf_density_vs_fp64": float(np.abs(rhoc12 - rho64).max())},
             "target_l1": TARGET_L1, "meets_target": l1 <= TARGET_L1,
             "l1_over_control": round(l1 / l1c, 3) if l1c > 0 else None}
 
