@@ -388,6 +388,7 @@ BATCH_T_END = 0.2
 BATCH_BYTES_PER_CELL_STAGE = 80    # TVD-RK2: stage 0 reads 32 B, stage 1 reads 64 B, each writes 32 B
 BATCH_PROFILE = "batch_stage_kernel_riemann_batch_256.md"
 KH_PROFILE = "stage_kernel_viscous_shear_layer_4096x4096.md"
+RIEMANN_PROFILE = "stage_kernel_laxliu_400x400.md"
 
 
 def run_batch(precision, count, poll_every=20, seed=BATCH_SEED, max_steps=20000, batch_only=False):
@@ -635,6 +636,20 @@ def main_riemann(args):
         "bit_identical_at_t_end": bool(np.array_equal(G.view(np.uint64), C.view(np.uint64)) and n == steps),
         "max_abs_diff": float(np.abs(G - C).max()),
     }
+    # rooflines over the solve's wall time per stage (launch gaps, halt polls
+    # and launches replayed after t = 0.3 included): algorithmic bytes 96 B
+    # per cell-stage (RK3 mean), VALU lane-ops per cell-stage from the
+    # committed counters of this kernel
+    stage_s = gpu_wall / (3 * steps)
+    gbs = BYTES_PER_CELL_STAGE_RK3 * cells / stage_s / 1e9
+    out["roofline"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "timed_over": "solve wall time",
+                       "kernel": "stage_kernel<double, false, false, TileShape<15, 16, 256, dual-face>>"}
+    lane_ops = profile_row(RIEMANN_PROFILE, "SQ_INSTS_VALU", col=3)
+    if lane_ops is not None:
+        v = lane_ops * cells / stage_s / 1e12
+        out["roofline"]["valu"] = {"achieved": round(v, 2), "peak": FP64_VALU_PEAK_T, "unit": "T lane-ops/s",
+                                   "frac": round(v / FP64_VALU_PEAK_T, 4), "lane_ops_per_cell_stage": lane_ops}
     print(json.dumps(out))
 
 

@@ -52,6 +52,13 @@ def test_shear_layer_profile_rows(bench):
     assert 2000 < lane_ops < 6000 and lane_ops_fp32 < lane_ops
 
 
+def test_laxliu_profile_rows(bench):
+    lane_ops = bench.profile_row(bench.RIEMANN_PROFILE, "SQ_INSTS_VALU", col=3)
+    fma = bench.profile_row(bench.RIEMANN_PROFILE, "SQ_INSTS_VALU_FMA_F64", col=3)
+    assert None not in (lane_ops, fma)
+    assert 2000 < lane_ops < 6000 and fma < lane_ops
+
+
 def test_missing_rows_read_as_none(bench):
     assert bench.profile_row(bench.BATCH_PROFILE, "no such row") is None
     assert bench.profile_row("no_such_profile.md", "traffic") is None
