@@ -39,6 +39,9 @@ Beside the headline the line carries the rest of the metric:
   kh           20 steps of the 4096^2 viscous shear layer in FP64 and
                FP32-bulk: state planes of 537 MB, beyond the Infinity Cache,
                so its stage kernel streams from HBM (roofline object);
+  laxliu       the Lax-Liu 400^2 solve to t = 0.3 (RK3, CFL 0.5, FP64) on
+               the GPU, with its rooflines (its bitwise comparison with the
+               CPU restatement is --workload riemann's);
   serving      the seeded request stream served with continuous batching:
                a 48-request burst and Poisson arrivals at 2 and 4 requests/s
                (32 requests), p50/p99 latency and solves/s, against the burst
@@ -539,6 +542,47 @@ def main_batch(args):
     print(json.dumps(out))
 
 
+def run_laxliu():
+    """The Lax-Liu configuration 3 solve on 400 x 400 (RK3, CFL 0.5, FP64, to
+    t = 0.3) on the GPU alone, timed after a warm-up that captures the CFL
+    graph it replays; Mcell-updates/s and the rooflines over its wall time per
+    stage (the bitwise comparison with the CPU restatement is
+    --workload riemann's)."""
+    cfg = cfs.config_riemann(400, rk=3)
+    U = cfs.riemann_quadrants(cfg)
+    s = cfs.Solver(cfg)
+    s.set_state(U)
+    s.advance(32, dt=0.0, cfl=CFL, t_end=0.3)
+    s.set_state(U)
+    t0 = time.perf_counter()
+    t, steps = s.advance(100000, dt=0.0, cfl=CFL, t_end=0.3)
+    wall = time.perf_counter() - t0
+    shape = s.tile_shape
+    s.close()
+    cells = cfg.nx * cfg.ny
+    out = {"workload": "riemann_laxliu3_400x400_rk3_cfl0.5_t0.3_fp64", "value": round(cells * 3 * steps / wall / 1e6, 2),
+           "unit": "Mcell-updates/s", "steps": steps, "t": t, "wall_s": round(wall, 4), "tile_shape": shape}
+    out["roofline"] = laxliu_roofline(cells, wall / (3 * steps))
+    return out
+
+
+def laxliu_roofline(cells, stage_s):
+    """Rooflines of the Lax-Liu solve over its wall time per stage (launch
+    gaps, halt polls and launches replayed after t = 0.3 included):
+    algorithmic bytes 96 B per cell-stage (RK3 mean), VALU lane-ops per
+    cell-stage from the committed counters of its kernel."""
+    gbs = BYTES_PER_CELL_STAGE_RK3 * cells / stage_s / 1e9
+    roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "timed_over": "solve wall time",
+            "kernel": "stage_kernel<double, false, false, TileShape<15, 16, 256, dual-face>>"}
+    lane_ops = profile_row(RIEMANN_PROFILE, "SQ_INSTS_VALU", col=3)
+    if lane_ops is not None:
+        v = lane_ops * cells / stage_s / 1e12
+        roof["valu"] = {"achieved": round(v, 2), "peak": FP64_VALU_PEAK_T, "unit": "T lane-ops/s",
+                        "frac": round(v / FP64_VALU_PEAK_T, 4), "lane_ops_per_cell_stage": lane_ops}
+    return roof
+
+
 def main_riemann(args):
     """The reference's own CPU-runnable case end to end: the Lax-Liu
     four-quadrant problem on 400 x 400, WENO5 + HLL + TVD-RK3, CFL 0.5, FP64,
@@ -636,20 +680,7 @@ def main_riemann(args):
         "bit_identical_at_t_end": bool(np.array_equal(G.view(np.uint64), C.view(np.uint64)) and n == steps),
         "max_abs_diff": float(np.abs(G - C).max()),
     }
-    # rooflines over the solve's wall time per stage (launch gaps, halt polls
-    # and launches replayed after t = 0.3 included): algorithmic bytes 96 B
-    # per cell-stage (RK3 mean), VALU lane-ops per cell-stage from the
-    # committed counters of this kernel
-    stage_s = gpu_wall / (3 * steps)
-    gbs = BYTES_PER_CELL_STAGE_RK3 * cells / stage_s / 1e9
-    out["roofline"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "timed_over": "solve wall time",
-                       "kernel": "stage_kernel<double, false, false, TileShape<15, 16, 256, dual-face>>"}
-    lane_ops = profile_row(RIEMANN_PROFILE, "SQ_INSTS_VALU", col=3)
-    if lane_ops is not None:
-        v = lane_ops * cells / stage_s / 1e12
-        out["roofline"]["valu"] = {"achieved": round(v, 2), "peak": FP64_VALU_PEAK_T, "unit": "T lane-ops/s",
-                                   "frac": round(v / FP64_VALU_PEAK_T, 4), "lane_ops_per_cell_stage": lane_ops}
+    out["roofline"] = laxliu_roofline(cells, gpu_wall / (3 * steps))
     print(json.dumps(out))
 
 
@@ -894,6 +925,7 @@ def main():
                     help="skip the FP32-bulk accuracy comparison at the workload's 2000-step horizon")
     ap.add_argument("--no-kh", action="store_true", help="skip the 4096^2 shear-layer leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-laxliu", action="store_true", help="skip the Lax-Liu 400^2 leg")
     ap.add_argument("--workload", choices=["cylinder", "batch", "serve", "kh", "riemann", "ibsetup"],
                     default="cylinder",
                     help="cylinder: the headline line; batch / serve: the batched-solves and serving measurements")
@@ -976,6 +1008,11 @@ def main():
     kh = None
     if not args.no_kh:
         kh = run_kh_short()
+    laxliu = None
+    if not args.no_laxliu:
+        laxliu = run_laxliu()
+        print("lax-liu 400^2 to t = 0.3: %.1f Mcell-updates/s (%d steps, %.4f s)"
+              % (laxliu["value"], laxliu["steps"], laxliu["wall_s"]), file=sys.stderr, flush=True)
     serving = None
     if not args.no_serve:
         serving = run_serving(cfs.PREC_FP32_BULK, 48, [0.0, 2.0, 4.0], rate_count=32)
@@ -1046,6 +1083,8 @@ def main():
             out["fp32_bulk"].update(acc)
     if kh is not None:
         out["kh"] = kh
+    if laxliu is not None:
+        out["laxliu"] = laxliu
     if serving is not None:
         out["serving"] = serving
     print(json.dumps(out))
