@@ -197,11 +197,18 @@ __device__ __forceinline__ double sqrt64_core(double x) {
     return __builtin_fma(d, h, g);
 }
 // IEEE sqrt: the short form on [2^-767, 2^1000] (nearly every value), else
-// the compiler's full sequence
+// the compiler's full sequence.  The empty asm keeps the second arm a branch,
+// skipped when no lane takes it: the full sequence is one intrinsic call until
+// instruction selection, and speculated as such it would run on every lane
+// beside the short form (about 20 more VALU instructions per root).
 __device__ __forceinline__ double sqrt64(double x) {
     double y;
-    if (x >= 0x1p-767 && x <= 0x1p+1000) y = sqrt64_core(x);
-    else y = __builtin_sqrt(x);
+    if (x >= 0x1p-767 && x <= 0x1p+1000) {
+        y = sqrt64_core(x);
+    } else {
+        asm volatile("" : "+v"(x));
+        y = __builtin_sqrt(x);
+    }
     return y;
 }
 
