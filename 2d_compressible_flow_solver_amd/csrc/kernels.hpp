@@ -236,14 +236,15 @@ template <> struct K<float> {
 // WOps<double, false> is plain IEEE division, used on the other tiles.  Both
 // give the IEEE quotient bit for bit on the tiles they run on.
 //
-// Division by the constants 3 and 6: q0 = x*z with z = RN(1/d), t = d*q0 - x,
-// q = q0 - t*z (two fmas).  For x == 0 or 2^-900 <= |x| <= 2^1000, q0 is within
-// 1 ulp of x/d, so t is exact (a small multiple of ulp(q0)); q0 - t*z differs
-// from x/d by at most |x/d|*2^-105, while x/d (d = 3*2^k) lies at least
-// 2^(e-53)/3 from every rounding midpoint of its binade 2^e, so the single
-// rounding of the last fma gives RN(x/d).  Signed zeros come out right without
-// a select: t is computed as d*q0 - x and negated, so an exact-zero residual is
-// -0 and q0 -+ 0 keeps the sign of q0 (= the sign of x when x is a zero).
+// Division by the constants 3 and 6: 1/d as z = RN(1/d) plus z' = RN(1/d - z)
+// (1/3 = 0x1.5555555555555p-2 + 0x1.5555555555555p-56 + 1.03e-33, 1/6 half of
+// each), q = RN(x z + RN(x z')): one multiply and one fma.  For
+// 2^-900 <= |x| <= 2^1000 (x z' normal), x z + RN(x z') differs from x/d by
+// at most |x| (|z + z' - 1/d| + |z'| 2^-53) < |x| 2^-107, while x/d
+// (d = 3*2^k) lies at least 2^(e-53)/3 > |x| 2^-59 from every rounding
+// midpoint of its binade 2^e (x/d is a multiple of 1/3 half-ulp, and never a
+// midpoint itself), so the fma's single rounding gives RN(x/d).  Zeros: x z'
+// and x z carry the sign of x, and their sum keeps it.
 //
 // Division by a variable: v_rcp_f64 and two Newton steps give r ~ 1/b, then
 // q = a*r, e = a - b*q (exact), RN(q + e*r) -- the sequence the compiler emits
@@ -258,18 +259,8 @@ template <> struct K<float> {
 // sequence computes for each of them).
 template <typename R, bool FAST> struct WOps;
 template <> struct WOps<double, true> {
-    __device__ static double d3(double x) {
-        const double z = 0x1.5555555555555p-2;
-        const double q0 = x * z;
-        const double t = __builtin_fma(q0, 3.0, -x);
-        return __builtin_fma(-t, z, q0);
-    }
-    __device__ static double d6(double x) {
-        const double z = 0x1.5555555555555p-3;
-        const double q0 = x * z;
-        const double t = __builtin_fma(q0, 6.0, -x);
-        return __builtin_fma(-t, z, q0);
-    }
+    __device__ static double d3(double x) { return __builtin_fma(x, 0x1.5555555555555p-2, x * 0x1.5555555555555p-56); }
+    __device__ static double d6(double x) { return __builtin_fma(x, 0x1.5555555555555p-3, x * 0x1.5555555555555p-57); }
     __device__ static double rcp(double b) {
         double r = __builtin_amdgcn_rcp(b);
         r = __builtin_fma(r, __builtin_fma(-b, r, 1.0), r);
@@ -306,6 +297,17 @@ template <bool FAST> struct WOps<float, FAST> {
 __device__ __forceinline__ bool fast_div_safe(double v) {
     const double a = __builtin_fabs(v);
     return v == 0.0 || (a >= 0x1p-900 && a <= 0x1p+200);
+}
+
+// Jiang-Shu smoothness indicator 13/12 s^2 + 1/4 t^2.  FP64 folds the
+// quarter into an fma: RN(A + t^2/4) equals the reference's RN(A + RN(t^2/4))
+// unless t^2/4 is subnormal, and then the two can differ only when the sum is
+// below 2^-967, where eps + beta -- the indicator's only use -- rounds to eps
+// either way.
+template <typename R>
+__device__ __forceinline__ R weno5_beta(R s, R t) {
+    if constexpr (sizeof(R) == 8) return __builtin_fma(t * t, 0.25, K<R>::c1312 * (s * s));
+    else return K<R>::c1312 * (s * s) + K<R>::q * (t * t);
 }
 
 // WENO5 value at the face from five cell values, v2 the upwind cell
@@ -402,13 +404,13 @@ __device__ __forceinline__ R weno5(R v0, R v1, R v2, R v3, R v4) {
     using D = WOps<R, FAST>;
     const R s0 = D::sub_pow2(v0, R(2), v1) + v2;
     const R t0 = D::sub_pow2(v0, R(4), v1) + R(3) * v2;
-    const R b0 = C::c1312 * (s0 * s0) + C::q * (t0 * t0);
+    const R b0 = weno5_beta<R>(s0, t0);
     const R s1 = D::sub_pow2(v1, R(2), v2) + v3;
     const R t1 = v1 - v3;
-    const R b1 = C::c1312 * (s1 * s1) + C::q * (t1 * t1);
+    const R b1 = weno5_beta<R>(s1, t1);
     const R s2 = D::sub_pow2(v2, R(2), v3) + v4;
     const R t2 = D::sub_pow2(R(3) * v2, R(4), v3) + v4;
-    const R b2 = C::c1312 * (s2 * s2) + C::q * (t2 * t2);
+    const R b2 = weno5_beta<R>(s2, t2);
     const R e0 = C::eps + b0, e1 = C::eps + b1, e2 = C::eps + b2;
     const R a0 = D::div(C::d0, e0 * e0);
     const R a1 = D::div(C::d1, e1 * e1);
