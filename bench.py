@@ -83,10 +83,14 @@ BYTES_PER_CELL_STAGE = 97.0
 BYTES_PER_CELL_STAGE_RK3 = 96.0   # RK3 mean without the cell-type byte (no immersed body)
 
 
+DUAL_FACE_SHAPE = 3   # shape 0's tile, each thread computing its x-face and its y-face in one pass
+
+
 def kernel_name(prec, limiter, shape):
     ti, tj, nt = cfs.tile_shape_info(shape)
-    return "stage_kernel<%s, false, %s, TileShape<%d, %d, %d>>" % (
-        "double" if prec == cfs.PREC_FP64 else "float", "true" if limiter else "false", ti, tj, nt)
+    return "stage_kernel<%s, false, %s, TileShape<%d, %d, %d%s>>" % (
+        "double" if prec == cfs.PREC_FP64 else "float", "true" if limiter else "false", ti, tj, nt,
+        ", true" if shape == DUAL_FACE_SHAPE else "")
 
 
 def shape_report(r):
@@ -97,16 +101,18 @@ def shape_report(r):
     return out
 
 
-def profile_figure(key):
-    """A figure of the FP64 stage kernel measured with rocprofv3 --pmc, from
-    the committed summary: traffic_bytes_per_launch (FETCH_SIZE + WRITE_SIZE,
-    scaled by the copy-kernel calibration) or valu_lane_ops_per_cell_stage
-    (SQ_INSTS_VALU x 64 / cells); None when absent."""
+def profile_figure(key, shape=0):
+    """A figure of the FP64 stage kernel of tile shape `shape` measured with
+    rocprofv3 --pmc, from the committed summary: traffic_bytes_per_launch
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration) or
+    valu_lane_ops_per_cell_stage (SQ_INSTS_VALU x 64 / cells), keyed
+    `<key>_shape<n>` for shapes other than 0; None when absent."""
     p = os.path.join(ROOT, "profiles", "stage_kernel_limiter_fp64_cylinder_mach3_2048x1024_counters.txt")
     if not os.path.exists(p):
         return None
+    name = key if shape == 0 else "%s_shape%d" % (key, shape)
     for line in open(p):
-        if line.startswith(key + ":"):
+        if line.startswith(name + ":"):
             return float(line.split(":", 1)[1])
     return None
 
@@ -574,7 +580,7 @@ def laxliu_roofline(cells, stage_s):
     gbs = BYTES_PER_CELL_STAGE_RK3 * cells / stage_s / 1e9
     roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "timed_over": "solve wall time",
-            "kernel": "stage_kernel<double, false, false, TileShape<15, 16, 256, dual-face>>"}
+            "kernel": kernel_name(cfs.PREC_FP64, False, DUAL_FACE_SHAPE)}
     lane_ops = profile_row(RIEMANN_PROFILE, "SQ_INSTS_VALU", col=3)
     if lane_ops is not None:
         v = lane_ops * cells / stage_s / 1e12
@@ -1047,13 +1053,13 @@ def main():
                    "ib_interpolation": "bilinear_positive", "finite_at_end": r64["finite"],
                    "tile_shape": shape_report(r64)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profile_figure("traffic_bytes_per_launch"),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profile_figure("traffic_bytes_per_launch", r64["shape"]),
                      "kernel": kernel_name(cfs.PREC_FP64, True, r64["shape"]),
                      "kernel_ms": round(r64["kernel_ms"], 5),
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,
     }
-    lane_ops = profile_figure("valu_lane_ops_per_cell_stage")
+    lane_ops = profile_figure("valu_lane_ops_per_cell_stage", r64["shape"])
     if lane_ops and r64["kernel_ms"] > 0:
         # the bound that binds: FP64 VALU issue (the kernel's VALU lane-ops per
         # cell-stage from the committed counters, at the measured launch duration)

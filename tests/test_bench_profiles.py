@@ -29,6 +29,25 @@ def test_headline_counters(bench):
     assert 2000 < lane_ops < 6000
 
 
+def test_headline_counters_of_the_dual_face_shape(bench):
+    # the autotuner may pick the dual-face shape for the headline workload:
+    # its kernel has its own counters and its own name on the bench line
+    traffic = bench.profile_figure("traffic_bytes_per_launch", bench.DUAL_FACE_SHAPE)
+    lane_ops = bench.profile_figure("valu_lane_ops_per_cell_stage", bench.DUAL_FACE_SHAPE)
+    assert traffic is not None and lane_ops is not None
+    alg = bench.BYTES_PER_CELL_STAGE * bench.NX * bench.NY
+    assert 0.9 < traffic / alg < 1.5
+    assert 2000 < lane_ops < 6000
+    assert lane_ops != bench.profile_figure("valu_lane_ops_per_cell_stage")
+    assert bench.profile_figure("valu_lane_ops_per_cell_stage", 1) is None
+
+
+def test_kernel_names_distinguish_the_tile_shapes(bench):
+    names = {bench.kernel_name(bench.cfs.PREC_FP64, True, k) for k in range(bench.cfs.NUM_TILE_SHAPES)}
+    assert len(names) == bench.cfs.NUM_TILE_SHAPES
+    assert bench.kernel_name(bench.cfs.PREC_FP64, True, bench.DUAL_FACE_SHAPE).endswith("256, true>>")
+
+
 def test_batch_profile_rows(bench):
     ratio = bench.profile_row(bench.BATCH_PROFILE, "traffic / algorithmic")
     lane_ops = bench.profile_row(bench.BATCH_PROFILE, "VALU lane-ops per cell-stage")
