@@ -1,9 +1,10 @@
 /* Checks two division forms of the FP64 stage kernel against IEEE division,
  * bit for bit.
  * 1. Division of a double by 3 or 6 as
- *      q0 = x*z;  t = fma(q0, d, -x);  q = fma(-t, z, q0)   (z = RN(1/d)),
- *    signed zeros included: random doubles over the normal range, integers,
- *    and x = d*y +- a few ulp (quotients next to rounding boundaries).
+ *      q = fma(x, z, x*z')   (z = RN(1/d), z' = RN(1/d - z)),
+ *    signed zeros included: random doubles with 2^-900 <= |x| <= 2^1000,
+ *    integers, and x = d*y +- a few ulp (quotients next to rounding
+ *    boundaries).
  * 2. Division a/b by a reciprocal refined with two Newton steps,
  *      r = fma(r, fma(-b, r, 1), r) (twice);  q = a*r;  q = fma(fma(-b, q, a), r, q),
  *    from a starting reciprocal with a relative error up to 2^-22 (the
@@ -20,11 +21,7 @@ static uint64_t s = 0x9E3779B97F4A7C15ull;
 static uint64_t next(void) { s ^= s << 13; s ^= s >> 7; s ^= s << 17; return s; }
 static double bits(uint64_t b) { double d; memcpy(&d, &b, 8); return d; }
 
-static double fastdiv(double x, double d, double z) {
-    double q0 = x * z;
-    double t = fma(q0, d, -x);
-    return fma(-t, z, q0);
-}
+static double fastdiv(double x, double z, double zl) { return fma(x, z, x * zl); }
 
 /* starting reciprocal: 1/b with a random relative error in [-2^-22, 2^-22] */
 static double rough_rcp(double b, uint64_t u) {
@@ -46,16 +43,16 @@ static double rand_in(uint64_t b, int e_lo, int e_hi) {   /* positive, exponent 
 
 int main(int argc, char** argv) {
     long n = argc > 1 ? atol(argv[1]) : 10000000;
-    const double ds[2] = {3.0, 6.0};
+    const double ds[2] = {3.0, 6.0}, zls[2] = {0x1.5555555555555p-56, 0x1.5555555555555p-57};
     long bad = 0;
     for (int k = 0; k < 2; ++k) {
-        const double d = ds[k], z = 1.0 / d;
+        const double d = ds[k], z = 1.0 / d, zl = zls[k];
         for (long i = 0; i < n; ++i) {
             double x;
             uint64_t b = next();
             switch (i & 3) {
-                case 0: {   /* any normal double with |x| >= 2^-1000 */
-                    uint64_t e = 23 + (b >> 11) % (2046 - 23);
+                case 0: {   /* any double with 2^-900 <= |x| < 2^1001 */
+                    uint64_t e = 123 + (b >> 11) % (2024 - 123);
                     x = bits((b & 0x800FFFFFFFFFFFFFull) | (e << 52));
                     break;
                 }
@@ -70,7 +67,7 @@ int main(int argc, char** argv) {
                 }
                 default: x = bits((b & 0x800FFFFFFFFFFFFFull) | (0x3FFull << 52)) * 8.0; break;
             }
-            double a = x / d, f = fastdiv(x, d, z);
+            double a = x / d, f = fastdiv(x, z, zl);
             if (memcmp(&a, &f, 8) != 0) {
                 if (bad < 10) printf("mismatch d=%g x=%a: %a vs %a\n", d, x, a, f);
                 ++bad;
@@ -78,7 +75,7 @@ int main(int argc, char** argv) {
         }
         double zz[2] = {0.0, -0.0};
         for (int t = 0; t < 2; ++t) {
-            double a = zz[t] / d, f = fastdiv(zz[t], d, z);
+            double a = zz[t] / d, f = fastdiv(zz[t], z, zl);
             if (memcmp(&a, &f, 8) != 0) ++bad;
         }
     }

@@ -1,8 +1,9 @@
 """Native (C) checks of arithmetic shortcuts used by the kernels.
 
 const_div_check.c: the division-free x/3, x/6 of the FP64 stage kernel
-(q0 = x*z; t = fma(q0, d, -x); q = fma(-t, z, q0)) equals IEEE x/d bit for bit on
-random normal-range doubles, integers, quotients next to rounding boundaries
+(q = fma(x, z, x*z'), 1/d split into z = RN(1/d) and z' = RN(1/d - z)) equals
+IEEE x/d bit for bit on random doubles with 2^-900 <= |x| <= 2^1000, integers,
+quotients next to rounding boundaries
 and signed zeros; and the WENO5 weight division through a reciprocal refined
 by two Newton steps (started with a relative error up to 2^-22) equals IEEE a/b
 over the operand ranges the weights take.
